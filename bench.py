@@ -1,0 +1,151 @@
+#!/usr/bin/env python3
+"""Throughput of the anisotropic-SPH step on one MI355X.
+
+Workload: the 256^3 = 16,777,216-particle moist step (32 x 32 x 8 km, 4:1 anisotropic kernel
+metric, supersaturated 1-2 km layer, Gaussian-correlated theta noise, dt = 0.2 s), synthetic
+initial state from the seeded generator, resident in HBM before timing starts. A "step" is one
+KDK step of the reference loop (asph_h100.cu:958-1078): displacement-gated sort, kick + drift,
+density + EOS + warm-rain microphysics, momentum + theta update + kick + sponge, rain
+sedimentation, and the per-step CFL/sort-gating readback. dt is held fixed at 0.2 s (the CFL
+controller monitors only).
+
+Prints one JSON line: particle-steps/s, the HBM-roofline fraction of the dominant kernel from
+HIP events on the library's stream, and the CPU oracle on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E peak [GB/s]
+# Algorithmic bytes per particle of each reference stage (reference data flow, float4 = 16 B):
+#   kick 48, drift 48, density 32, eos_micro 56, momentum 84, theta_update 36, kick 48,
+#   sponge 8.8, rain 8, max_vel 16 (total 384.8); a re-sort adds 212 (keys 20, 3 radix passes
+#   4 + 16*3, gather 108, ranges 32).
+B_STAGE = dict(kick_drift=48 + 48, density_eos=32 + 56, momentum_kdk=84 + 36 + 48 + 8.8 + 16,
+               rain=8.0)
+B_STEP = 384.8
+B_SORT = 212.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(cfg, st, dt, steps):
+    """The FP32 oracle on the host cores: `steps` steps of the same workload."""
+    from oracle.oracle import Oracle
+
+    o = Oracle(cfg)
+    o.set_state(st)
+    o.sort()
+    o.force(dt)
+    t0 = time.perf_counter()
+    for k in range(steps):
+        o.step(dt, 0.2 * k, do_sort=(k == 0))
+    wall = time.perf_counter() - t0
+    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=cfg.n * steps / wall, unit="particle-steps/s", cores=cores, kind="port",
+                sample=f"{steps} steps (one with a re-sort) of the {cfg.nx}^3 workload, "
+                       f"FP32 oracle, OpenMP, {wall:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=300)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--preset", type=int, default=3)
+    ap.add_argument("--n", type=int, default=0, help="override lattice edge (cubic)")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on one GPU only: the step is a single-GPU kernel set "
+              "(multi-GPU slab decomposition is a design, not an implementation)", file=sys.stderr)
+        sys.exit(2)
+
+    import sph_cuda_amd as s
+
+    cfg, ic, dt, _ = s.preset(args.preset)
+    if args.n:
+        cfg.nx = cfg.ny = cfg.nz = args.n
+    log(f"[bench] preset {args.preset}: {cfg.nx}x{cfg.ny}x{cfg.nz} = {cfg.n} particles, dt {dt}")
+    t0 = time.perf_counter()
+    st = s.ic_generate(cfg, ic)
+    log(f"[bench] initial state generated in {time.perf_counter() - t0:.1f} s")
+
+    sim = s.Sim(cfg)
+    sim.set_state(st)
+    rc, ws = sim.run(max_steps=args.warmup, dt=dt, fixed_dt=True)
+    if rc != 0:
+        raise SystemExit(f"warmup aborted rc={rc}")
+    sim.synchronize()
+    log(f"[bench] warmup {args.warmup} steps done ({ws.sorts} sorts)")
+
+    sim.set_profiling(True)
+    sim.kernel_times()   # reset
+    sim.synchronize()
+    t0 = time.perf_counter()
+    rc, ts = sim.run(max_steps=args.steps, dt=dt, fixed_dt=True)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    if rc != 0:
+        raise SystemExit(f"timed run aborted rc={rc}")
+    kt = sim.kernel_times()
+    log(f"[bench] timed {args.steps} steps in {wall:.3f} s, {ts.sorts} sorts; kernels: {kt}")
+
+    n = cfg.n
+    value = n * args.steps / wall
+    # dominant kernel by total time
+    dom = max(("kick_drift", "density_eos", "momentum_kdk", "rain"), key=lambda k: kt[k][0])
+    ms_total, cnt = kt[dom]
+    avg_s = ms_total / 1e3 / max(cnt, 1)
+    bytes_launch = B_STAGE[dom] * n
+    achieved = bytes_launch / avg_s / 1e9
+    step_alg_gbs = (B_STEP * n * args.steps + B_SORT * n * ts.sorts) / wall / 1e9
+
+    out = {
+        "metric": "particle-steps/s & % HBM peak (plain vs batched); adjoint/forward time ratio",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic (seeded initial state, no dataset)",
+        "config": {
+            "workload": f"plain moist anisotropic step, {cfg.nx}x{cfg.ny}x{cfg.nz} = {n} particles, "
+                        f"dx=dy={cfg.dx} m dz={cfg.dz} m, h=1.3 cells, dt={dt} s fixed",
+            "particles": n, "sorts_in_timed_steps": int(ts.sorts),
+            "hbm_frac_step_alg": step_alg_gbs / HBM_PEAK_GBS,
+            "step_alg_bytes_per_particle": B_STEP, "sort_alg_bytes_per_particle": B_SORT,
+            "kernel_ms_per_launch": {k: (v[0] / max(v[1], 1)) for k, v in kt.items()},
+            "kernel_launches": {k: v[1] for k, v in kt.items()},
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+            "alg_bytes_per_particle": B_STAGE[dom], "traffic": None,
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu_baseline:
+        log("[bench] CPU oracle sample ...")
+        out["cpu_baseline"] = cpu_baseline(cfg, st, dt, args.cpu_steps)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

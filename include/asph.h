@@ -1,0 +1,276 @@
+/*
+ * asph.h - C ABI of the MI355X-native anisotropic-SPH atmospheric step.
+ *
+ * The reference (asph_h100.cu) has no library interface: its "step interface" is the set of
+ * __global__ kernel signatures driven by the loop body of main() (asph_h100.cu:958-1078), its
+ * state layout is the set of device arrays allocated at asph_h100.cu:828-851 and its external
+ * contracts are the CLI (asph_h100.cu:756-784) and the dump file (asph_h100.cu:743-747,
+ * 1106-1120). Every entry point below names the reference code it replaces.
+ *
+ * Conventions (all entry points):
+ *   - return ASPH_OK (0) on success, ASPH_ERR (1) on an argument or device error,
+ *     ASPH_CFL_ABORT (2) when the closed-loop CFL controller aborts
+ *     (reference: CK -> exit(1) at asph_h100.cu:41-43, `return 1` at 787, `return 2` at 1065);
+ *     the library never calls exit(); asph_last_error() returns the message of the last failure
+ *     on the calling thread;
+ *   - a handle owns its device memory and is used by one host thread at a time; handles are
+ *     independent; all work of a handle is issued in order on the stream given at creation
+ *     (NULL = the legacy default stream, as the reference);
+ *   - host pointers passed in are borrowed for the duration of the call only;
+ *   - particle state uses the reference layout: pos, vel, thq as float4 [x,y,z,w] arrays
+ *     (thq = [theta, qv, qc, 0]), rho as float, all of length N = nx*ny*nz, in the current
+ *     (sorted) particle order.
+ */
+#ifndef ASPH_H
+#define ASPH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ASPH_OK 0
+#define ASPH_ERR 1
+#define ASPH_CFL_ABORT 2
+
+/* Physics / behaviour switches (asph_config.flags). */
+#define ASPH_F_SURFACE_FLUX   (1u << 0) /* bulk surface flux, asph_h100.cu:606-626 */
+#define ASPH_F_SPONGE         (1u << 1) /* Rayleigh sponge, asph_h100.cu:518-530 */
+#define ASPH_F_AUTOCONV       (1u << 2) /* autoconversion onto the rain grid, asph_h100.cu:389-394 */
+#define ASPH_F_ACCRETION      (1u << 3) /* Kessler accretion (no reference counterpart) */
+#define ASPH_F_RAIN_EVAP      (1u << 4) /* Kessler rain evaporation (no reference counterpart) */
+#define ASPH_F_SEDIMENT       (1u << 5) /* rain sedimentation, asph_h100.cu:577-599 */
+#define ASPH_F_GATHER_ACC     (1u << 6) /* permute acc with the sort (the reference does not) */
+#define ASPH_F_MIN_IMAGE      (1u << 7) /* minimum-image pair separations in x,y (the reference
+                                           uses raw differences, asph_h100.cu:315-317, 454) */
+#define ASPH_F_SORT_EVERY_STEP (1u << 8) /* re-sort every step instead of displacement gating */
+
+/* Background moisture profile qv_ref(z) used by the ICs and by the buoyancy reference. */
+#define ASPH_QV_DRY      0 /* qv_ref = 0 */
+#define ASPH_QV_REF_EXP  1 /* qv0*exp(-z/hs)*exp(-max(z-zt,0)/ht), asph_h100.cu:149-153 */
+#define ASPH_QV_RH_TABLE 2 /* RH(z) piecewise linear in z times qsat(T_ref(z), p_ref(z)) */
+
+#define ASPH_RH_KNOTS 8
+
+typedef struct asph_config {
+    /* lattice / cell grid: cells = particles = nx*ny*nz (asph_h100.cu:790) */
+    int32_t nx, ny, nz;
+    /* cell sizes [m] = the kernel metric per axis (reference constexpr DX,DY,DZ, :48) */
+    float dx, dy, dz;
+    /* smoothing length in cell units; support radius 2h cells (reference H_SUP=1, :52) */
+    float h;
+    /* cells searched either side; 0 selects ceil(2h) (reference STENCIL=int(2*H_SUP), :54) */
+    int32_t stencil;
+    /* reference sounding (:71-73) */
+    float th0, n2, cs;
+    /* background moisture (:111, :149-153) */
+    int32_t qv_mode;
+    float qv0, qv_hscale, qv_ztop, qv_htop;
+    int32_t rh_n;
+    float rh_z[ASPH_RH_KNOTS], rh_v[ASPH_RH_KNOTS];
+    /* dissipation (:84-91) */
+    float alpha_av, h_av;
+    float sponge_sigma, sponge_frac, dthdt_cap;
+    /* surface flux (:100-105) */
+    float c_h, c_e, beta_moist, v_bg, dts_skin, ramp;
+    /* warm rain (:116-120) plus Kessler accretion rate k2 [1/s] */
+    float qc_auto, rate_auto, k_accr, v_rain, theta_cap, qv_cap;
+    /* closed-loop CFL (:730-734) */
+    float cfl, dt_cap;
+    int32_t max_trip, strict_cfl;
+    uint32_t flags;
+} asph_config;
+
+/* Initial-condition recipe (the reference's kernel_init, asph_h100.cu:210-227, generalised:
+ * seeded jitter, bubble, correlated noise, initial cloud). */
+typedef struct asph_ic_spec {
+    uint64_t seed;
+    int32_t reference_init;   /* 1: reproduce kernel_init exactly (cell centres + hash noise) */
+    float ref_noise;          /* kernel_init's `noise` amplitude [K] (Config::noise, :741) */
+    float jitter;             /* particle jitter U(+-jitter*d) per axis, in cells */
+    float bubble_dth;         /* bubble amplitude [K]: dth*cos^2(pi r / 2R) for r < R */
+    float bubble_r;           /* R [m] */
+    float bubble_x, bubble_y, bubble_z; /* centre [m] */
+    float noise_sigma;        /* correlated theta noise std [K] (0 = none) */
+    float noise_len;          /* Gaussian correlation length [m] */
+    float qc_init;            /* initial cloud water inside the bubble radius [kg/kg] */
+    float white_noise;        /* white theta noise std [K] (0 = none) */
+} asph_ic_spec;
+
+typedef struct asph_t asph_t;
+
+/* Run-loop statistics (the reference's shutdown summary, asph_h100.cu:1163-1175). */
+typedef struct asph_run_stats {
+    int64_t steps;
+    int64_t sorts;
+    int64_t trips;
+    int64_t dumps;
+    double simt;
+    float dt;            /* dt after the last controller update */
+    float dt_required;   /* last CFL-required dt */
+    float umax[3];       /* last per-axis max |v| */
+    double wall_s;       /* host wall time of the loop */
+} asph_run_stats;
+
+typedef struct asph_run_args {
+    double sim_end;       /* stop when simt >= sim_end (ignored if <= 0) */
+    int64_t max_steps;    /* stop after this many steps (ignored if <= 0) */
+    float dt;             /* initial dt; <= 0 selects the controller's cap */
+    int32_t fixed_dt;     /* 1: keep dt fixed, CFL only monitors and gates the sort */
+    double dump_every;    /* sim seconds between dumps (<= 0: none) */
+    int32_t dump_sx, dump_sy, dump_sz;
+    const char* outdir;   /* dump directory (NULL: no dumps) */
+    int32_t validate;     /* FP64 audit at every dump, asph_h100.cu:1126-1144 */
+    int32_t verbose;      /* print per-dump lines like asph_h100.cu:1121-1124 */
+} asph_run_args;
+
+/* --- configuration ------------------------------------------------------------------- */
+
+/* sizeof(asph_config) etc., for binding-layout checks. */
+size_t asph_sizeof_config(void);
+size_t asph_sizeof_ic_spec(void);
+size_t asph_sizeof_run_args(void);
+size_t asph_sizeof_run_stats(void);
+
+/* The reference's compile-time constants (asph_h100.cu:45-120) and Config defaults (:728-742)
+ * as a runtime configuration, with the reference's behaviour flags (no acc gather, no
+ * minimum image, surface flux and sponge on, autoconversion + sedimentation only). */
+int asph_config_reference(asph_config* cfg, int32_t nx, int32_t ny, int32_t nz);
+
+/* Workload presets: 1 = 32^3 dry bubble, 2 = 128^3 moist 2:1, 3 = 256^3 moist 4:1,
+ * 4 = 64^3 adjoint window, 5 = ensemble member template (nx,ny,nz overwritten by caller).
+ * Fills cfg and (if non-NULL) ic, dt and steps. */
+int asph_config_preset(int32_t which, asph_config* cfg, asph_ic_spec* ic, float* dt,
+                       int32_t* steps);
+
+/* Validates cfg (Morton limit 1024 cells per axis, asph_h100.cu:785-788; stencil fits). */
+int asph_config_check(const asph_config* cfg);
+
+/* --- initial conditions (host) ------------------------------------------------------------ */
+
+/* Generates the initial state on the host into caller arrays (pos4, vel4, thq4: 4*N floats,
+ * rho: N floats), replacing kernel_init (asph_h100.cu:210-227, 891). */
+int asph_ic_generate(const asph_config* cfg, const asph_ic_spec* ic, float* pos4, float* vel4,
+                     float* thq4, float* rho);
+
+/* --- handle ---------------------------------------------------------------------------- */
+
+/* Allocates the device state (asph_h100.cu:835-862). stream: a hipStream_t or NULL. */
+int asph_create(const asph_config* cfg, void* stream, asph_t** out);
+int asph_destroy(asph_t* h);
+const char* asph_last_error(void);
+int asph_get_config(const asph_t* h, asph_config* cfg);
+int64_t asph_num_particles(const asph_t* h);
+
+/* Uploads a particle state (host pointers) and clears acc, dthdt, prs, the rain grid, the
+ * precipitation field and the sort-gating clock (asph_h100.cu:884-889). */
+int asph_set_state(asph_t* h, const float* pos4, const float* vel4, const float* thq4,
+                   const float* rho);
+/* Same with device pointers (e.g. torch tensors); acc4 may be NULL (then zeroed). */
+int asph_set_state_device(asph_t* h, const float* pos4, const float* vel4, const float* thq4,
+                          const float* rho, const float* acc4);
+/* Downloads state to host arrays; any pointer may be NULL. prs and dthdt are the values of
+ * the last force evaluation. */
+int asph_get_state(asph_t* h, float* pos4, float* vel4, float* thq4, float* rho, float* acc4,
+                   float* prs, float* dthdt);
+/* Device pointers of the current state buffers (valid until the next sort or step). */
+int asph_state_device_ptrs(asph_t* h, float** pos4, float** vel4, float** thq4, float** rho,
+                           float** acc4);
+/* Sets acc (host float4 array), e.g. for the first kick after an external state upload. */
+int asph_set_acc(asph_t* h, const float* acc4);
+
+/* Rain grid [kg per cell] (reference rainG, asph_h100.cu:841) and accumulated surface
+ * precipitation [kg/m^2] (precip2d, :849). Stored as exact 64-bit fixed point internally
+ * (units of particle mass * 2^-40); these convert to/from float. */
+int asph_get_rain(asph_t* h, float* rain_kg, float* precip_kgm2);
+int asph_set_rain(asph_t* h, const float* rain_kg);
+/* Raw fixed-point rain state (int64, units pmass*2^-40), for exact tests. */
+int asph_get_rain_fx(asph_t* h, int64_t* rain, int64_t* precip, int64_t* clipped);
+
+/* --- the step ---------------------------------------------------------------------------- */
+
+/* Neighbour structure: Morton keys, stable LSD radix sort, gather of pos/vel/thq/rho (and acc
+ * with ASPH_F_GATHER_ACC), cell [start,end) table and surface list
+ * (asph_h100.cu:962-976; kernels :229-278). */
+int asph_sort(asph_t* h);
+/* Results of the last sort: keys before sorting, sorted keys, permutation (sorted index ->
+ * pre-sort index), cell ranges (2 ints per cell, empty = [0,0)), and the surface list (as a
+ * set; its order is unspecified, like the reference's atomic append). Any may be NULL. */
+int asph_get_sort(asph_t* h, uint32_t* keys, uint32_t* keys_sorted, uint32_t* perm,
+                  int32_t* cell_range, int32_t* surf_list, int32_t* surf_count);
+int asph_sort_end_bit(const asph_t* h);
+
+/* Initial force evaluation (asph_h100.cu:913-920): density (lagged Shepard), EOS +
+ * microphysics with time step dt, momentum. Writes rho, thq, prs, acc, dthdt. */
+int asph_force(asph_t* h, float dt);
+
+/* Stage entry points, each one fused kernel of the step (reference kernels in brackets):
+ *   kick_drift   [kernel_kick(dt/2) :535, kernel_drift :545]
+ *   density_eos  [kernel_surface_flux :606 (flag), kernel_density :286, kernel_eos_micro :340,
+ *                 + Kessler accretion / evaporation (flags)]
+ *   momentum_kdk [kernel_momentum :415, kernel_theta_update :562, kernel_kick(dt/2),
+ *                 kernel_sponge :518, kernel_max_vel :631]
+ *   rain         [kernel_rain_adv :577]
+ * simt is the simulated time at the start of the step (surface skin temperature, :989-992). */
+int asph_kick_drift(asph_t* h, float dt);
+int asph_density_eos(asph_t* h, float dt, double simt);
+int asph_momentum_kdk(asph_t* h, float dt);
+int asph_rain(asph_t* h, float dt);
+
+/* One KDK step in the reference order (asph_h100.cu:962-1038): an optional sort, then the four
+ * stages above. Asynchronous: no host synchronisation. */
+int asph_step(asph_t* h, float dt, double simt, int32_t do_sort);
+
+/* Per-axis max |v| after the last momentum_kdk (kernel_max_vel, asph_h100.cu:1043-1046).
+ * Synchronises the stream. */
+int asph_max_vel(asph_t* h, float out[3]);
+/* Rescales |v| <= vmax (kernel_clamp_vel, asph_h100.cu:669-675). */
+int asph_clamp_vel(asph_t* h, float vmax);
+
+/* FP64 conservation audit (kernel_audit, asph_h100.cu:650-668): out = [KE, PE, IE, water,
+ * pz, mass]; out[6] = rain-grid + precipitation water [kg] (closing the budget the reference
+ * audit leaves open). Synchronises. */
+int asph_audit(asph_t* h, double out[7]);
+
+/* The reference's main loop (asph_h100.cu:958-1078) with its displacement-gated sort and
+ * closed-loop CFL controller (trip counter, velocity clamp, abort -> ASPH_CFL_ABORT), and
+ * optional dumps. Continues from the handle's clock (asph_set_state resets it). */
+int asph_run(asph_t* h, const asph_run_args* args, asph_run_stats* stats);
+/* Simulated time and sort-gating clock of the handle. */
+int asph_get_clock(const asph_t* h, double* simt, double* since_sort, float* sort_dt);
+
+/* --- dumps -------------------------------------------------------------------------------- */
+
+/* Coarse block means of [theta,qv,qc,u,v,w] (kernel_coarse_dump, asph_h100.cu:680-703) into
+ * host out[cx*cy*cz*6], block index cxx + cx*(cyy + cy*czz); empty blocks are 0. */
+int asph_coarse_dump(asph_t* h, int32_t sx, int32_t sy, int32_t sz, float* out);
+/* Per-column [mean theta, mean qv, max qc, max |w|] (kernel_column_diag, :704-723). */
+int asph_column_diag(asph_t* h, float* out4);
+/* Writes "<outdir>/asph_t%06.0fs.bin" in the reference format (asph_h100.cu:1106-1120):
+ * 68-byte header <2I10i5f> then cx*cy*cz*6 float32. */
+int asph_write_dump(asph_t* h, const char* outdir, int32_t sx, int32_t sy, int32_t sz,
+                    double simt, float dt, char* path_out, size_t path_cap);
+
+/* --- profiling --------------------------------------------------------------------------- */
+
+/* Kernel ids for asph_kernel_times. */
+#define ASPH_K_KICK_DRIFT   0
+#define ASPH_K_DENSITY_EOS  1
+#define ASPH_K_MOMENTUM_KDK 2
+#define ASPH_K_RAIN         3
+#define ASPH_K_SORT         4
+#define ASPH_K_COUNT        5
+/* When enabled, every launch of the step kernels is bracketed by HIP events on the handle's
+ * stream; asph_kernel_times synchronises and returns per-kernel total ms and launch counts
+ * since the last reset. */
+int asph_set_profiling(asph_t* h, int32_t enable);
+int asph_kernel_times(asph_t* h, double ms[ASPH_K_COUNT], int64_t counts[ASPH_K_COUNT]);
+int asph_synchronize(asph_t* h);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASPH_H */

@@ -1,0 +1,43 @@
+// asph_kernels.h - host-side launchers of the gfx950 kernels (asph_kernels.hip, asph_sort.hip).
+#pragma once
+
+#include "asph_internal.h"
+
+namespace asph {
+
+void launch_kick_drift(const DevParams& P, hipStream_t s, float4* pos, float4* vel,
+                       const float4* acc, int64_t* dep, float dt);
+void launch_density_eos(const DevParams& P, hipStream_t s, const float4* pos, const float* rho_old,
+                        float* rho_new, float4* thq, float* prs, const uint32_t* cstart,
+                        const uint32_t* keys_sorted, const int64_t* rain, int64_t* dep,
+                        uint32_t* maxv, float dt, float auto_f, float Ts, float qs_sfc, int do_eos);
+void launch_momentum_kdk(const DevParams& P, hipStream_t s, const float4* pos, const float4* vel,
+                         float4* thq, const float* rho, const float* prs, const uint32_t* cstart,
+                         float4* acc, float* dthdt, float4* vel_out, uint32_t* maxv, float dt,
+                         int mode);
+void launch_rain(const DevParams& P, hipStream_t s, const int64_t* rain_in, const int64_t* dep,
+                 int64_t* rain_out, int64_t* precip, unsigned long long* clipped, float frac,
+                 int sediment);
+void launch_rain_merge(hipStream_t s, int n, int64_t* rain, int64_t* dep);
+void launch_gather(hipStream_t s, int N, const uint32_t* perm, const float4* pA, float4* pB,
+                   const float4* vA, float4* vB, const float4* tA, float4* tB, const float* rA,
+                   float* rB, const float4* aA, float4* aB);
+void launch_cell_start(hipStream_t s, int N, uint32_t nkeys, const uint32_t* keys, uint32_t* cstart);
+void launch_clamp_vel(hipStream_t s, int N, float4* vel, float vmax);
+void launch_audit(const DevParams& P, hipStream_t s, const float4* pos, const float4* vel,
+                  const float4* thq, const int64_t* rain, const int64_t* dep, const int64_t* precip,
+                  double* tot);
+void launch_coarse_dump(const DevParams& P, hipStream_t s, const uint32_t* cstart, const float4* vel,
+                        const float4* thq, float* out, int sx, int sy, int sz, int cx, int cy, int cz);
+void launch_column_diag(const DevParams& P, hipStream_t s, const uint32_t* cstart, const float4* thq,
+                        const float4* vel, float4* out);
+
+int radix_tile();
+// Sorts the Morton keys of pos over bits [0, P.end_bit). keys0 receives the unsorted keys;
+// (kA,vA) and (kB,vB) are ping-pong buffers; bh, bo hold 256*nblk counts, dtot 256 totals.
+// On return *keys_sorted and *perm point into kA/vA or kB/vB.
+void launch_radix_sort(const DevParams& P, hipStream_t s, const float4* pos, uint32_t* keys0,
+                       uint32_t* kA, uint32_t* vA, uint32_t* kB, uint32_t* vB, uint32_t* bh,
+                       uint32_t* bo, uint32_t* dtot, uint32_t** keys_sorted, uint32_t** perm);
+
+}  // namespace asph

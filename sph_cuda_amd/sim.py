@@ -1,0 +1,233 @@
+"""Python mirror of the reference's step interface over the C ABI (include/asph.h).
+
+`Sim` owns one device handle. Its methods are the reference's step pieces
+(asph_h100.cu:958-1078): `sort` (keys, radix sort, gather, cell table), `force` (initial
+density/EOS/momentum), the four fused stages, `step`, and `run` (the closed-loop CFL driver).
+States are exchanged as numpy arrays in the reference layout: pos, vel, thq of shape (N, 4)
+float32 and rho of shape (N,) float32.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass
+
+import numpy as np
+
+from . import _lib as L
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(L.FP)
+
+
+def preset(which: int):
+    """Returns (Config, ICSpec, dt, steps) of a workload preset (1..5)."""
+    lib = L.load()
+    cfg, ic = L.Config(), L.ICSpec()
+    dt, steps = C.c_float(), C.c_int32()
+    L.check(lib.asph_config_preset(which, C.byref(cfg), C.byref(ic), C.byref(dt), C.byref(steps)),
+            "asph_config_preset")
+    return cfg, ic, float(dt.value), int(steps.value)
+
+
+def reference_config(nx: int, ny: int, nz: int) -> L.Config:
+    lib = L.load()
+    cfg = L.Config()
+    L.check(lib.asph_config_reference(C.byref(cfg), nx, ny, nz), "asph_config_reference")
+    return cfg
+
+
+def check_config(cfg: L.Config) -> None:
+    L.check(L.load().asph_config_check(C.byref(cfg)), "asph_config_check")
+
+
+@dataclass
+class State:
+    pos: np.ndarray
+    vel: np.ndarray
+    thq: np.ndarray
+    rho: np.ndarray
+
+    def copy(self) -> "State":
+        return State(self.pos.copy(), self.vel.copy(), self.thq.copy(), self.rho.copy())
+
+
+def ic_generate(cfg: L.Config, ic: L.ICSpec) -> State:
+    """Initial state on the host (replaces kernel_init, asph_h100.cu:210-227)."""
+    n = cfg.n
+    st = State(np.zeros((n, 4), np.float32), np.zeros((n, 4), np.float32),
+               np.zeros((n, 4), np.float32), np.zeros(n, np.float32))
+    L.check(L.load().asph_ic_generate(C.byref(cfg), C.byref(ic), _fp(st.pos), _fp(st.vel),
+                                      _fp(st.thq), _fp(st.rho)), "asph_ic_generate")
+    return st
+
+
+class Sim:
+    def __init__(self, cfg: L.Config, stream: int | None = None):
+        self.lib = L.load()
+        self.cfg = cfg.copy()
+        self.h = C.c_void_p()
+        L.check(self.lib.asph_create(C.byref(self.cfg), C.c_void_p(stream or 0), C.byref(self.h)),
+                "asph_create")
+        self.n = int(self.lib.asph_num_particles(self.h))
+
+    def close(self):
+        if self.h:
+            self.lib.asph_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # ---- state ------------------------------------------------------------------------
+    def set_state(self, st: State):
+        args = [np.ascontiguousarray(a, np.float32) for a in (st.pos, st.vel, st.thq, st.rho)]
+        L.check(self.lib.asph_set_state(self.h, *[_fp(a) for a in args]), "asph_set_state")
+
+    def get_state(self) -> State:
+        n = self.n
+        st = State(np.empty((n, 4), np.float32), np.empty((n, 4), np.float32),
+                   np.empty((n, 4), np.float32), np.empty(n, np.float32))
+        L.check(self.lib.asph_get_state(self.h, _fp(st.pos), _fp(st.vel), _fp(st.thq), _fp(st.rho),
+                                        None, None, None), "asph_get_state")
+        return st
+
+    def get_aux(self):
+        """(acc (N,4), prs (N,), dthdt (N,)) of the last force evaluation."""
+        n = self.n
+        acc, prs, dth = np.empty((n, 4), np.float32), np.empty(n, np.float32), np.empty(n, np.float32)
+        L.check(self.lib.asph_get_state(self.h, None, None, None, None, _fp(acc), _fp(prs), _fp(dth)),
+                "asph_get_state")
+        return acc, prs, dth
+
+    def set_acc(self, acc: np.ndarray):
+        a = np.ascontiguousarray(acc, np.float32)
+        L.check(self.lib.asph_set_acc(self.h, _fp(a)), "asph_set_acc")
+
+    def get_rain_fx(self):
+        ncell, ncol = self.n, self.cfg.nx * self.cfg.ny
+        r, p, c = np.empty(ncell, np.int64), np.empty(ncol, np.int64), C.c_int64()
+        L.check(self.lib.asph_get_rain_fx(self.h, r.ctypes.data_as(L.I64P), p.ctypes.data_as(L.I64P),
+                                          C.byref(c)), "asph_get_rain_fx")
+        return r, p, int(c.value)
+
+    def get_rain(self):
+        ncell, ncol = self.n, self.cfg.nx * self.cfg.ny
+        r, p = np.empty(ncell, np.float32), np.empty(ncol, np.float32)
+        L.check(self.lib.asph_get_rain(self.h, _fp(r), _fp(p)), "asph_get_rain")
+        return r, p
+
+    def set_rain(self, rain_kg: np.ndarray):
+        a = np.ascontiguousarray(rain_kg, np.float32)
+        L.check(self.lib.asph_set_rain(self.h, _fp(a)), "asph_set_rain")
+
+    # ---- neighbour structure ----------------------------------------------------------------
+    def sort(self):
+        L.check(self.lib.asph_sort(self.h), "asph_sort")
+
+    def get_sort(self):
+        n, ncell = self.n, self.n
+        keys, ks, perm = (np.empty(n, np.uint32) for _ in range(3))
+        cr = np.empty(2 * ncell, np.int32)
+        sl = np.empty(2 * self.cfg.nx * self.cfg.ny, np.int32)
+        sc = C.c_int32()
+        L.check(self.lib.asph_get_sort(self.h, keys.ctypes.data_as(L.U32P), ks.ctypes.data_as(L.U32P),
+                                       perm.ctypes.data_as(L.U32P), cr.ctypes.data_as(L.I32P),
+                                       sl.ctypes.data_as(L.I32P), C.byref(sc)), "asph_get_sort")
+        return dict(keys=keys, keys_sorted=ks, perm=perm, cell_range=cr.reshape(-1, 2),
+                    surf_list=sl[: sc.value].copy())
+
+    # ---- the step -------------------------------------------------------------------------------
+    def force(self, dt: float):
+        L.check(self.lib.asph_force(self.h, dt), "asph_force")
+
+    def kick_drift(self, dt: float):
+        L.check(self.lib.asph_kick_drift(self.h, dt), "asph_kick_drift")
+
+    def density_eos(self, dt: float, simt: float = 0.0):
+        L.check(self.lib.asph_density_eos(self.h, dt, simt), "asph_density_eos")
+
+    def momentum_kdk(self, dt: float):
+        L.check(self.lib.asph_momentum_kdk(self.h, dt), "asph_momentum_kdk")
+
+    def rain(self, dt: float):
+        L.check(self.lib.asph_rain(self.h, dt), "asph_rain")
+
+    def step(self, dt: float, simt: float = 0.0, do_sort: bool = False):
+        L.check(self.lib.asph_step(self.h, dt, simt, 1 if do_sort else 0), "asph_step")
+
+    def max_vel(self) -> np.ndarray:
+        out = np.empty(3, np.float32)
+        L.check(self.lib.asph_max_vel(self.h, _fp(out)), "asph_max_vel")
+        return out
+
+    def clamp_vel(self, vmax: float):
+        L.check(self.lib.asph_clamp_vel(self.h, vmax), "asph_clamp_vel")
+
+    def audit(self) -> np.ndarray:
+        out = np.empty(7, np.float64)
+        L.check(self.lib.asph_audit(self.h, out.ctypes.data_as(L.DP)), "asph_audit")
+        return out
+
+    def run(self, sim_end: float = 0.0, max_steps: int = 0, dt: float = 0.0, fixed_dt: bool = False,
+            dump_every: float = 0.0, dump_stride=(10, 10, 3), outdir: str | None = None,
+            validate: bool = False, verbose: bool = False):
+        """The reference's main loop (asph_h100.cu:958-1078). Returns (rc, RunStats)."""
+        a = L.RunArgs()
+        a.sim_end, a.max_steps, a.dt, a.fixed_dt = sim_end, max_steps, dt, int(fixed_dt)
+        a.dump_every = dump_every
+        a.dump_sx, a.dump_sy, a.dump_sz = dump_stride
+        a.outdir = outdir.encode() if outdir else None
+        a.validate, a.verbose = int(validate), int(verbose)
+        st = L.RunStats()
+        rc = self.lib.asph_run(self.h, C.byref(a), C.byref(st))
+        if rc not in (L.OK, L.CFL_ABORT):
+            L.check(rc, "asph_run")
+        return rc, st
+
+    def clock(self):
+        t, s, d = C.c_double(), C.c_double(), C.c_float()
+        L.check(self.lib.asph_get_clock(self.h, C.byref(t), C.byref(s), C.byref(d)), "asph_get_clock")
+        return t.value, s.value, d.value
+
+    # ---- dumps --------------------------------------------------------------------------------
+    def coarse_dump(self, sx: int, sy: int, sz: int) -> np.ndarray:
+        c = self.cfg
+        cx, cy, cz = -(-c.nx // sx), -(-c.ny // sy), -(-c.nz // sz)
+        out = np.empty(cx * cy * cz * 6, np.float32)
+        L.check(self.lib.asph_coarse_dump(self.h, sx, sy, sz, _fp(out)), "asph_coarse_dump")
+        return out.reshape(cz, cy, cx, 6)
+
+    def column_diag(self) -> np.ndarray:
+        out = np.empty(self.cfg.nx * self.cfg.ny * 4, np.float32)
+        L.check(self.lib.asph_column_diag(self.h, _fp(out)), "asph_column_diag")
+        return out.reshape(-1, 4)
+
+    def write_dump(self, outdir: str, stride, simt: float, dt: float) -> str:
+        buf = C.create_string_buffer(1024)
+        L.check(self.lib.asph_write_dump(self.h, outdir.encode(), stride[0], stride[1], stride[2],
+                                         simt, dt, buf, 1024), "asph_write_dump")
+        return buf.value.decode()
+
+    # ---- profiling ------------------------------------------------------------------------------
+    def set_profiling(self, on: bool):
+        L.check(self.lib.asph_set_profiling(self.h, 1 if on else 0), "asph_set_profiling")
+
+    def kernel_times(self):
+        ms = np.zeros(L.K_COUNT, np.float64)
+        cnt = np.zeros(L.K_COUNT, np.int64)
+        L.check(self.lib.asph_kernel_times(self.h, ms.ctypes.data_as(L.DP), cnt.ctypes.data_as(L.I64P)),
+                "asph_kernel_times")
+        return {L.KERNEL_NAMES[k]: (float(ms[k]), int(cnt[k])) for k in range(L.K_COUNT)}
+
+    def synchronize(self):
+        L.check(self.lib.asph_synchronize(self.h), "asph_synchronize")
