@@ -213,6 +213,8 @@ struct asph_t {
     uint32_t *bh = nullptr, *bo = nullptr, *dtot = nullptr, *cstart = nullptr;
     uint32_t *keys_sorted = nullptr, *perm = nullptr;
     uint32_t* maxv = nullptr;
+    float* disp = nullptr;           // drift since the last sort [cells per axis]
+    uint32_t* disp_slots = nullptr;  // per-step drift maxima (64 slots per axis)
     double* audit_buf = nullptr;
     std::vector<void*> allocs;
     bool sorted = false, initialized = false;
@@ -488,6 +490,8 @@ int asph_create(const asph_config* cfg, void* stream, asph_t** out) {
     rc |= h->alloc(&h->dtot, 256);
     rc |= h->alloc(&h->cstart, (size_t)h->nkeys + 1);
     rc |= h->alloc(&h->maxv, 3 * 64);
+    rc |= h->alloc(&h->disp, 4);
+    rc |= h->alloc(&h->disp_slots, 3 * 64);
     rc |= h->alloc(&h->audit_buf, 8);
     if (rc != ASPH_OK) { std::string e = g_err; asph_destroy(h); g_err = e; return ASPH_ERR; }
     *out = h;
@@ -523,6 +527,8 @@ static int reset_aux(asph_t* h, bool zero_acc) {
     HIPCHK(hipMemsetAsync(h->precip, 0, (size_t)h->ncol * sizeof(int64_t), h->stream));
     HIPCHK(hipMemsetAsync(h->clipped, 0, sizeof(unsigned long long), h->stream));
     HIPCHK(hipMemsetAsync(h->maxv, 0, 3 * 64 * sizeof(uint32_t), h->stream));
+    HIPCHK(hipMemsetAsync(h->disp, 0, 4 * sizeof(float), h->stream));
+    HIPCHK(hipMemsetAsync(h->disp_slots, 0, 3 * 64 * sizeof(uint32_t), h->stream));
     h->sorted = false;
     h->initialized = false;
     h->dep_pending = false;
@@ -566,6 +572,8 @@ int asph_get_state(asph_t* h, float* pos4, float* vel4, float* thq4, float* rho,
     if (prs) HIPCHK(hipMemcpyAsync(prs, h->prs, N * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     if (dthdt) HIPCHK(hipMemcpyAsync(dthdt, h->dthdt, N * sizeof(float), hipMemcpyDeviceToHost, h->stream));
     HIPCHK(hipStreamSynchronize(h->stream));
+    // vel.w carries p' internally; the reference layout has w = 0
+    if (vel4) for (size_t i = 0; i < N; ++i) vel4[4 * i + 3] = 0.f;
     return ASPH_OK;
 }
 
@@ -643,6 +651,7 @@ int asph_sort(asph_t* h) {
         std::swap(h->rho, h->rho2);
         if (ga) std::swap(h->acc, h->acc2);
         launch_cell_start(h->stream, (int)h->N, h->nkeys, h->keys_sorted, h->cstart);
+        launch_disp_reset(h->stream, h->disp, h->disp_slots);
     }
     LAUNCHCHK();
     h->sorted = true;
@@ -686,16 +695,29 @@ int asph_get_sort(asph_t* h, uint32_t* keys, uint32_t* keys_sorted, uint32_t* pe
     return ASPH_OK;
 }
 
+static MomArgs mom_args(asph_t* h, float dt, int mode) {
+    MomArgs A;
+    A.pos = h->pos; A.vel = h->vel; A.thq = h->thq; A.rho = h->rho; A.prs = h->prs;
+    A.cstart = h->cstart; A.acc = h->acc; A.dthdt = h->dthdt; A.vel_out = h->vel2;
+    A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
+    A.dt = dt; A.mode = mode;
+    return A;
+}
+
 static int density_eos_impl(asph_t* h, float dt, double simt, int bits) {
     if (!h->sorted) return fail("asph_sort must run before the neighbour kernels");
     const DevParams& P = h->P;
     float auto_f = 1.0f - asphm::fexp(-P.rate_auto * dt);
     float Ts = 0.f, qs = 0.f;
     if (bits & 2) sfc_params(P, simt, Ts, qs);
+    DensArgs A;
+    A.pos = h->pos; A.rho_old = h->rho; A.rho_new = h->rho2; A.thq = h->thq; A.prs = h->prs;
+    A.vel = h->vel; A.cstart = h->cstart; A.keys_sorted = h->keys_sorted; A.rain = h->rain;
+    A.dep = h->dep; A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
+    A.dt = dt; A.auto_f = auto_f; A.Ts = Ts; A.qs_sfc = qs; A.bits = bits;
     {
         ProfScope ps(h, ASPH_K_DENSITY_EOS);
-        launch_density_eos(P, h->stream, h->pos, h->rho, h->rho2, h->thq, h->prs, h->cstart,
-                           h->keys_sorted, h->rain, h->dep, h->maxv, dt, auto_f, Ts, qs, bits);
+        launch_density_eos(P, h->stream, A);
     }
     LAUNCHCHK();
     std::swap(h->rho, h->rho2);
@@ -709,8 +731,7 @@ int asph_force(asph_t* h, float dt) {
     if (density_eos_impl(h, dt, 0.0, 1) != ASPH_OK) return ASPH_ERR;
     launch_rain_merge(h->stream, (int)h->ncell, h->rain, h->dep);
     h->dep_pending = false;
-    launch_momentum_kdk(h->P, h->stream, h->pos, h->vel, h->thq, h->rho, h->prs, h->cstart, h->acc,
-                        h->dthdt, h->vel2, h->maxv, dt, 0);
+    launch_momentum_kdk(h->P, h->stream, mom_args(h, dt, 0));
     LAUNCHCHK();
     h->initialized = true;
     return ASPH_OK;
@@ -720,7 +741,7 @@ int asph_kick_drift(asph_t* h, float dt) {
     if (!h) return fail("null handle");
     {
         ProfScope ps(h, ASPH_K_KICK_DRIFT);
-        launch_kick_drift(h->P, h->stream, h->pos, h->vel, h->acc, h->dep, dt);
+        launch_kick_drift(h->P, h->stream, h->pos, h->vel, h->acc, h->dep, h->disp, h->disp_slots, dt);
     }
     LAUNCHCHK();
     h->dep_pending = false;
@@ -738,8 +759,7 @@ int asph_momentum_kdk(asph_t* h, float dt) {
     if (!h->sorted) return fail("asph_sort must run before the neighbour kernels");
     {
         ProfScope ps(h, ASPH_K_MOMENTUM_KDK);
-        launch_momentum_kdk(h->P, h->stream, h->pos, h->vel, h->thq, h->rho, h->prs, h->cstart,
-                            h->acc, h->dthdt, h->vel2, h->maxv, dt, 1);
+        launch_momentum_kdk(h->P, h->stream, mom_args(h, dt, 1));
     }
     LAUNCHCHK();
     std::swap(h->vel, h->vel2);

@@ -5,16 +5,47 @@
 
 namespace asph {
 
+struct DensArgs {
+    const float4* pos;
+    const float* rho_old;
+    float* rho_new;
+    float4* thq;
+    float* prs;
+    float4* vel;                 // vel.w receives p' for the momentum kernel
+    const uint32_t* cstart;
+    const uint32_t* keys_sorted;
+    const int64_t* rain;
+    int64_t* dep;
+    uint32_t* maxv;
+    const float* disp;
+    const uint32_t* disp_slots;
+    float dt, auto_f, Ts, qs_sfc;
+    int bits;                    // bit 0: EOS + microphysics, bit 1: surface flux
+};
+
+struct MomArgs {
+    const float4* pos;
+    const float4* vel;           // (vx, vy, vz, p')
+    float4* thq;
+    const float* rho;
+    const float* prs;
+    const uint32_t* cstart;
+    float4* acc;
+    float* dthdt;
+    float4* vel_out;
+    uint32_t* maxv;
+    const float* disp;
+    const uint32_t* disp_slots;
+    float dt;
+    int mode;                    // 0: force evaluation, 1: step
+};
+
 void launch_kick_drift(const DevParams& P, hipStream_t s, float4* pos, float4* vel,
-                       const float4* acc, int64_t* dep, float dt);
-void launch_density_eos(const DevParams& P, hipStream_t s, const float4* pos, const float* rho_old,
-                        float* rho_new, float4* thq, float* prs, const uint32_t* cstart,
-                        const uint32_t* keys_sorted, const int64_t* rain, int64_t* dep,
-                        uint32_t* maxv, float dt, float auto_f, float Ts, float qs_sfc, int do_eos);
-void launch_momentum_kdk(const DevParams& P, hipStream_t s, const float4* pos, const float4* vel,
-                         float4* thq, const float* rho, const float* prs, const uint32_t* cstart,
-                         float4* acc, float* dthdt, float4* vel_out, uint32_t* maxv, float dt,
-                         int mode);
+                       const float4* acc, int64_t* dep, float* disp, uint32_t* disp_slots, float dt);
+void launch_disp_reset(hipStream_t s, float* disp, uint32_t* disp_slots);
+void launch_density_eos(const DevParams& P, hipStream_t s, const DensArgs& A);
+void launch_momentum_kdk(const DevParams& P, hipStream_t s, const MomArgs& A);
+int brick_count(const DevParams& P);
 void launch_rain(const DevParams& P, hipStream_t s, const int64_t* rain_in, const int64_t* dep,
                  int64_t* rain_out, int64_t* precip, unsigned long long* clipped, float frac,
                  int sediment);

@@ -10,8 +10,7 @@
 // 4-byte start table in key order instead of the reference's int2 cellRange in row-major
 // order (asph_h100.cu:257-278), written with coalesced stores and no clearing pass.
 //
-// Summation order inside every neighbour loop is the reference's jz -> jy -> jx -> j
-// (asph_h100.cu:306-327, 444-495), so results do not depend on the launch shape.
+// The two neighbour-loop kernels live in asph_nbr.hip.
 #include "asph_kernels.h"
 
 namespace asph {
@@ -20,28 +19,6 @@ namespace {
 
 constexpr int TPB = 256;
 
-__device__ __forceinline__ void min_image(const DevParams& P, float& rx, float& ry) {
-    if (P.flags & ASPH_F_MIN_IMAGE) {
-        if (rx > P.hLx) rx -= P.Lx; else if (rx < -P.hLx) rx += P.Lx;
-        if (ry > P.hLy) ry -= P.Ly; else if (ry < -P.hLy) ry += P.Ly;
-    }
-}
-
-// Stencil of particle p: z range clipped at the walls (asph_h100.cu:301-303) and the spread
-// (Morton) bits of every wrapped x column.
-template <int S> struct Stencil {
-    int ix, iy, iz, z0, z1;
-    uint32_t px[2 * S + 1];
-    __device__ __forceinline__ Stencil(const DevParams& P, float x, float y, float z) {
-        cell_trunc(P, x, y, z, ix, iy, iz);
-        bool interior = (iz >= S && iz < P.nz - S);
-        z0 = interior ? iz - S : max(iz - S, 0);
-        z1 = interior ? iz + S : min(iz + S, P.nz - 1);
-#pragma unroll
-        for (int d = 0; d < 2 * S + 1; ++d) px[d] = part1by2((uint32_t)wrapi(ix - S + d, P.nx));
-    }
-};
-
 // ---------------------------------------------------------------------------------------------
 // kick(dt/2) + drift, asph_h100.cu:535-561. Also clears the per-step rain deposit grid
 // (consumed by k_rain of the previous step).
@@ -49,9 +26,12 @@ template <int S> struct Stencil {
 __global__ __launch_bounds__(TPB) void k_kick_drift(DevParams P, float4* __restrict__ pos,
                                                     float4* __restrict__ vel,
                                                     const float4* __restrict__ acc,
-                                                    int64_t* __restrict__ dep, float hdt, float dt) {
+                                                    int64_t* __restrict__ dep,
+                                                    uint32_t* __restrict__ disp_slots, float hdt,
+                                                    float dt) {
     int i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= P.N) return;
+    float mx = 0.f, my = 0.f, mz = 0.f;
+    if (i < P.N) {
     float4 v = vel[i];
     float4 a = acc[i];
     v.x = __builtin_fmaf(a.x, hdt, v.x);
@@ -71,256 +51,40 @@ __global__ __launch_bounds__(TPB) void k_kick_drift(DevParams P, float4* __restr
     pos[i] = p;
     vel[i] = v;
     if (i < P.ncell) dep[i] = 0;
+    // drift bound of this step in cells per axis (|dx| <= |v dt|, also across a wrap or a wall)
+    mx = fabsf(v.x * dt) * P.inv_dx;
+    my = fabsf(v.y * dt) * P.inv_dy;
+    mz = fabsf(v.z * dt) * P.inv_dz;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, o));
+        my = fmaxf(my, __shfl_xor(my, o));
+        mz = fmaxf(mz, __shfl_xor(mz, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        int slot = (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) & 63;
+        atomicMax(&disp_slots[slot], __float_as_uint(mx));
+        atomicMax(&disp_slots[64 + slot], __float_as_uint(my));
+        atomicMax(&disp_slots[128 + slot], __float_as_uint(mz));
+    }
 }
 
-// ---------------------------------------------------------------------------------------------
-// Surface flux (asph_h100.cu:606-626) + lagged-Shepard density (:286-333) + EOS, saturation
-// adjustment, autoconversion (:340-402) + Kessler accretion and rain evaporation.
-// The flux is applied first: the density loop does not read thq, so the reference's order
-// flux -> density -> eos is kept.
-// ---------------------------------------------------------------------------------------------
-template <int S>
-__global__ __launch_bounds__(TPB) void k_density_eos(
-    DevParams P, const float4* __restrict__ pos, const float* __restrict__ rho_old,
-    float* __restrict__ rho_new, float4* __restrict__ thq, float* __restrict__ prs,
-    const uint32_t* __restrict__ cstart, const uint32_t* __restrict__ keys_sorted,
-    const int64_t* __restrict__ rain, int64_t* __restrict__ dep, uint32_t* __restrict__ maxv,
-    float dt, float auto_f, float Ts, float qs_sfc, int do_eos) {
-    // do_eos: bit 0 = EOS + microphysics after the density, bit 1 = surface flux first
-    int i = blockIdx.x * TPB + threadIdx.x;
-    if (blockIdx.x == 0 && threadIdx.x < 3 * 64) maxv[threadIdx.x] = 0u;
-    if (i >= P.N) return;
-    const float4 pi = pos[i];
-    Stencil<S> st(P, pi.x, pi.y, pi.z);
-
-    float S1 = 0.f, S2 = 0.f;
-    for (int jz = st.z0; jz <= st.z1; ++jz) {
-        uint32_t kz = part1by2((uint32_t)jz) << 2;
-        for (int dy = 0; dy < 2 * S + 1; ++dy) {
-            uint32_t kyz = kz | (part1by2((uint32_t)wrapi(st.iy - S + dy, P.ny)) << 1);
+// Drift bookkeeping: disp[a] accumulates the per-step maxima (slots) since the last sort; the
+// neighbour kernels read disp + max(slots) as the bound on how far a particle has left the
+// cell it is listed in. reset != 0 clears both (after a sort).
+__global__ void k_disp_fold(float* __restrict__ disp, uint32_t* __restrict__ slots, int reset) {
+    int t = threadIdx.x;   // 64 threads
+    float m[3];
 #pragma unroll
-            for (int dx = 0; dx < 2 * S + 1; ++dx) {
-                uint32_t k = kyz | st.px[dx];
-                uint32_t s = cstart[k], e = cstart[k + 1];
-                for (uint32_t j = s; j < e; ++j) {
-                    float4 pj = pos[j];
-                    float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                    min_image(P, rx, ry);
-                    float dxm = rx * P.inv_dx, dym = ry * P.inv_dy, dzm = rz * P.inv_dz;
-                    float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
-                    if (r2 < P.lim) {
-                        float W = fker(sqrtf(r2) * P.inv_h);
-                        S1 += W;
-                        S2 += W * (1.0f / fmaxf(rho_old[j], 1e-3f));
-                    }
-                }
-            }
-        }
-    }
-    const float z = pi.z;
-    const float ex = exner_ref(P, z);
-    const float pref = pres_of_exner(P, ex);
-    const float rref = pref / ((R_D * theta_ref(P, z)) * ex);
-    float rr = S2 > 1e-20f ? S1 / S2 : rref;
-    float rnew = fminf(fmaxf(rr, 0.5f * rref), 2.0f * rref);
-    rho_new[i] = rnew;
-    if (!do_eos) return;
-
-    float4 t = thq[i];
-    if ((do_eos & 2) && (keys_sorted[i] & 0x24924924u) == 0u && z < P.dz) {
-        float Pi = asphm::fpow(fmaxf(pref, 1.0f) / P0, P.rd_cp);
-        float Ta = t.x * Pi;
-        t.x = t.x + (((P.c_h * P.v_bg) * (Ts - Ta)) * dt) / P.dz;
-        t.y = t.y + ((((P.beta_moist * P.c_e) * P.v_bg) * (qs_sfc - t.y)) * dt) / P.dz;
-        t.x = fminf(fmaxf(t.x, 100.f), P.theta_cap);
-        t.y = fminf(fmaxf(t.y, 0.f), P.qv_cap);
-    }
-
-    // EOS about the hydrostatic reference, asph_h100.cu:353-362
-    const float r = fmaxf(rnew, 1e-3f);
-    float dd = r - rref;
-    dd = fminf(fmaxf(dd, -0.3f * rref), 0.3f * rref);
-    const float ppr = P.cs2 * dd;
-    const float pfl = fmaxf(pref + ppr, 1.0f);
-    const float Pi = asphm::fpow(pfl / P0, P.rd_cp);
-    float th = t.x, qv = t.y, qc = t.z;
-    float T = th * Pi;
-    const float LvCp = L_V / CP;
-    float qs = qsat(T, pfl);
-    if (qv > qs || qc > 0.f) {  // Newton saturation adjustment, :370-385
-        float dqc = 0.f;
-        for (int it = 0; it < 3; ++it) {
-            float Tn = T + LvCp * dqc;
-            float esn = esat(Tn);
-            float qsn = qsat(Tn, pfl);
-            float f = (qv - dqc) - qsn;
-            float fp = -1.f - dqsdT(Tn, esn, pfl) * LvCp;
-            dqc -= f / fp;
-            dqc = fminf(dqc, qv);
-            dqc = fmaxf(dqc, -qc);
-        }
-        qv -= dqc;
-        qc += dqc;
-        th += (L_V * dqc) / (CP * Pi);
-        T += LvCp * dqc;
-    }
-    int64_t dfx = 0;
-    if ((P.flags & ASPH_F_AUTOCONV) && qc > P.qc_auto) {  // :389-394
-        float dqr = (qc - P.qc_auto) * auto_f;
-        qc -= dqr;
-        dfx += to_fx(dqr);
-    }
-    int cid = -1;
-    if (P.flags & (ASPH_F_ACCRETION | ASPH_F_RAIN_EVAP)) {
-        cid = cell_of(P, pi.x, pi.y, pi.z);
-        float rf = rain_fx_to_float(rain[cid]);
-        float qr = (rf * P.mm) / r;
-        if ((P.flags & ASPH_F_ACCRETION) && qc > 0.f && qr > 0.f) {
-            float qcn = qc / (1.0f + (P.k_accr * dt) * asphm::fpow(qr, 0.875f));
-            dfx += to_fx(qc - qcn);
-            qc = qcn;
-        }
-        if ((P.flags & ASPH_F_RAIN_EVAP) && qr > 0.f) {
-            float qs2 = qsat(T, pfl);
-            if (qv < qs2) {
-                float rc = 0.001f * r;
-                float rq = rc * qr;
-                float C = 1.6f + 124.9f * asphm::fpow(rq, 0.2046f);
-                float E = (((1.0f - qv / qs2) * C) * asphm::fpow(rq, 0.525f)) /
-                          ((5.4e5f + 2.55e8f / (pfl * qs2)) * rc);
-                float dqe = fminf(fminf(E * dt, qr), qs2 - qv);
-                qv += dqe;
-                th -= (L_V * dqe) / (CP * Pi);
-                dfx -= to_fx(dqe);
-            }
-        }
-    }
-    if (dfx != 0) {
-        if (cid < 0) cid = cell_of(P, pi.x, pi.y, pi.z);
-        atomicAdd((unsigned long long*)&dep[cid], (unsigned long long)dfx);
-    }
-    th = fminf(fmaxf(th, 100.0f), P.theta_cap);
-    qv = fminf(fmaxf(qv, 0.0f), P.qv_cap);
-    qc = fmaxf(qc, 0.0f);
-    thq[i] = make_float4(th, qv, qc, 0.f);
-    prs[i] = ppr;
-}
-
-// ---------------------------------------------------------------------------------------------
-// Momentum (asph_h100.cu:415-511). mode 0 (force evaluation, :919): write acc and dthdt.
-// mode 1 (step): write acc, then theta update (:562-571), second half kick, sponge (:518-530)
-// into vel_out, and the per-axis max |v| (:631-647) reduced per wavefront and merged with
-// atomicMax over 64 slots per axis.
-// ---------------------------------------------------------------------------------------------
-template <int S>
-__global__ __launch_bounds__(TPB) void k_momentum_kdk(
-    DevParams P, const float4* __restrict__ pos, const float4* __restrict__ vel,
-    float4* __restrict__ thq, const float* __restrict__ rho, const float* __restrict__ prs,
-    const uint32_t* __restrict__ cstart, float4* __restrict__ acc, float* __restrict__ dthdt,
-    float4* __restrict__ vel_out, uint32_t* __restrict__ maxv, float dt, int mode) {
-    int i = blockIdx.x * TPB + threadIdx.x;
-    float mx = 0.f, my = 0.f, mz = 0.f;
-    if (i < P.N) {
-        const float4 pi = pos[i];
-        const float4 vi = vel[i];
-        const float4 ti = thq[i];
-        Stencil<S> st(P, pi.x, pi.y, pi.z);
-        const float rhoi = fmaxf(rho[i], 1e-3f);
-        const float inv_rhoi = 1.0f / rhoi;
-        const float pri = prs[i];
-        float ax = 0.f, ay = 0.f, az = 0.f, avx = 0.f, avy = 0.f, avz = 0.f;
-        for (int jz = st.z0; jz <= st.z1; ++jz) {
-            uint32_t kz = part1by2((uint32_t)jz) << 2;
-            for (int dy = 0; dy < 2 * S + 1; ++dy) {
-                uint32_t kyz = kz | (part1by2((uint32_t)wrapi(st.iy - S + dy, P.ny)) << 1);
+    for (int a = 0; a < 3; ++a) {
+        float v = __uint_as_float(slots[a * 64 + t]);
 #pragma unroll
-                for (int dx = 0; dx < 2 * S + 1; ++dx) {
-                    uint32_t k = kyz | st.px[dx];
-                    uint32_t s = cstart[k], e = cstart[k + 1];
-                    for (uint32_t j = s; j < e; ++j) {
-                        if ((int)j == i) continue;
-                        float4 pj = pos[j];
-                        float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                        min_image(P, rx, ry);
-                        float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
-                        float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
-                        if (r2t >= P.lim || r2t < 1e-12f) continue;
-                        float q = sqrtf(r2t) * P.inv_h;
-                        float kk = kgrad(P, q);
-                        float gx = (kk * dxt) * P.inv_dx;
-                        float gy = (kk * dyt) * P.inv_dy;
-                        float gz = (kk * dzt) * P.inv_dz;
-                        float rhoj = fmaxf(rho[j], 1e-3f);
-                        float inv_rhoj = 1.0f / rhoj;
-                        float prj = prs[j];
-                        float dp = (prj - pri) * (inv_rhoi * inv_rhoj);
-                        float4 vj = vel[j];
-                        float vdx = vi.x - vj.x, vdy = vi.y - vj.y, vdz = vi.z - vj.z;
-                        float vdotr = __builtin_fmaf(vdz, rz, __builtin_fmaf(vdy, ry, vdx * rx));
-                        float visc = 0.f;
-                        if (vdotr < 0.f) {
-                            float rp2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
-                            float rbar = 0.5f * (rhoi + rhoj);
-                            visc = (P.visc_c * vdotr) / (((rp2 + P.eta2) * (rbar * rbar)) * rhoj);
-                        }
-                        float br = dp + visc;
-                        ax = __builtin_fmaf(-br, gx, ax);
-                        ay = __builtin_fmaf(-br, gy, ay);
-                        az = __builtin_fmaf(-br, gz, az);
-                        avx = __builtin_fmaf(-visc, gx, avx);
-                        avy = __builtin_fmaf(-visc, gy, avy);
-                        avz = __builtin_fmaf(-visc, gz, avz);
-                    }
-                }
-            }
-        }
-        ax *= P.mm; ay *= P.mm; az *= P.mm;
-        avx *= P.mm; avy *= P.mm; avz *= P.mm;
-        const float z = pi.z;
-        const float ex = exner_ref(P, z);
-        const float thr = theta_ref(P, z);
-        float thv = ti.x * (1.0f + 0.608f * ti.y);
-        float thvr = thr * (1.0f + 0.608f * qv_ref(P, z));
-        az = az + GRAV * ((thv - thvr) / thvr - ti.z);
-        acc[i] = make_float4(ax, ay, az, 0.f);
-        float Pi = asphm::fpow(fmaxf(pres_of_exner(P, ex) + pri, 1.0f) / P0, P.rd_cp);
-        float dth = -__builtin_fmaf(vi.z, avz, __builtin_fmaf(vi.y, avy, vi.x * avx)) / (CP * Pi);
-        dth = fminf(fmaxf(dth, -P.dthdt_cap), P.dthdt_cap);
-        if (mode == 0) {
-            dthdt[i] = dth;
-        } else {
-            float4 t = ti;
-            t.x = fminf(fmaxf(__builtin_fmaf(dth, dt, t.x), 100.0f), P.theta_cap);
-            thq[i] = t;
-            const float hdt = 0.5f * dt;
-            float4 v = vi;
-            v.x = __builtin_fmaf(ax, hdt, v.x);
-            v.y = __builtin_fmaf(ay, hdt, v.y);
-            v.z = __builtin_fmaf(az, hdt, v.z);
-            if ((P.flags & ASPH_F_SPONGE) && z >= P.sponge_zs) {
-                float tt = (z - P.sponge_zs) / P.sponge_depth;
-                float sg = (P.sponge_sigma * tt) * tt;
-                v.z *= 1.0f / (1.0f + sg * dt);
-            }
-            vel_out[i] = v;
-            mx = fabsf(v.x); my = fabsf(v.y); mz = fabsf(v.z);
-        }
+        for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+        m[a] = v;
+        slots[a * 64 + t] = 0u;
     }
-    if (mode != 0) {
-        // wave64 max, then one atomic per wave into slot (block % 64) of each axis
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            mx = fmaxf(mx, __shfl_xor(mx, o));
-            my = fmaxf(my, __shfl_xor(my, o));
-            mz = fmaxf(mz, __shfl_xor(mz, o));
-        }
-        if ((threadIdx.x & 63) == 0) {
-            int slot = blockIdx.x & 63;
-            atomicMax(&maxv[slot], __float_as_uint(mx));
-            atomicMax(&maxv[64 + slot], __float_as_uint(my));
-            atomicMax(&maxv[128 + slot], __float_as_uint(mz));
-        }
-    }
+    if (t < 3) disp[t] = reset ? 0.f : disp[t] + m[t];
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -507,34 +271,13 @@ inline int nblocks(int64_t n) { return (int)((n + TPB - 1) / TPB); }
 // launchers
 // ---------------------------------------------------------------------------------------------
 void launch_kick_drift(const DevParams& P, hipStream_t s, float4* pos, float4* vel,
-                       const float4* acc, int64_t* dep, float dt) {
-    k_kick_drift<<<nblocks(P.N), TPB, 0, s>>>(P, pos, vel, acc, dep, 0.5f * dt, dt);
+                       const float4* acc, int64_t* dep, float* disp, uint32_t* disp_slots, float dt) {
+    k_disp_fold<<<1, 64, 0, s>>>(disp, disp_slots, 0);
+    k_kick_drift<<<nblocks(P.N), TPB, 0, s>>>(P, pos, vel, acc, dep, disp_slots, 0.5f * dt, dt);
 }
 
-void launch_density_eos(const DevParams& P, hipStream_t s, const float4* pos, const float* rho_old,
-                        float* rho_new, float4* thq, float* prs, const uint32_t* cstart,
-                        const uint32_t* keys_sorted, const int64_t* rain, int64_t* dep,
-                        uint32_t* maxv, float dt, float auto_f, float Ts, float qs_sfc, int do_eos) {
-    dim3 g(nblocks(P.N));
-    switch (P.S) {
-        case 1: k_density_eos<1><<<g, TPB, 0, s>>>(P, pos, rho_old, rho_new, thq, prs, cstart, keys_sorted, rain, dep, maxv, dt, auto_f, Ts, qs_sfc, do_eos); break;
-        case 2: k_density_eos<2><<<g, TPB, 0, s>>>(P, pos, rho_old, rho_new, thq, prs, cstart, keys_sorted, rain, dep, maxv, dt, auto_f, Ts, qs_sfc, do_eos); break;
-        case 3: k_density_eos<3><<<g, TPB, 0, s>>>(P, pos, rho_old, rho_new, thq, prs, cstart, keys_sorted, rain, dep, maxv, dt, auto_f, Ts, qs_sfc, do_eos); break;
-        default: k_density_eos<4><<<g, TPB, 0, s>>>(P, pos, rho_old, rho_new, thq, prs, cstart, keys_sorted, rain, dep, maxv, dt, auto_f, Ts, qs_sfc, do_eos); break;
-    }
-}
-
-void launch_momentum_kdk(const DevParams& P, hipStream_t s, const float4* pos, const float4* vel,
-                         float4* thq, const float* rho, const float* prs, const uint32_t* cstart,
-                         float4* acc, float* dthdt, float4* vel_out, uint32_t* maxv, float dt,
-                         int mode) {
-    dim3 g(nblocks(P.N));
-    switch (P.S) {
-        case 1: k_momentum_kdk<1><<<g, TPB, 0, s>>>(P, pos, vel, thq, rho, prs, cstart, acc, dthdt, vel_out, maxv, dt, mode); break;
-        case 2: k_momentum_kdk<2><<<g, TPB, 0, s>>>(P, pos, vel, thq, rho, prs, cstart, acc, dthdt, vel_out, maxv, dt, mode); break;
-        case 3: k_momentum_kdk<3><<<g, TPB, 0, s>>>(P, pos, vel, thq, rho, prs, cstart, acc, dthdt, vel_out, maxv, dt, mode); break;
-        default: k_momentum_kdk<4><<<g, TPB, 0, s>>>(P, pos, vel, thq, rho, prs, cstart, acc, dthdt, vel_out, maxv, dt, mode); break;
-    }
+void launch_disp_reset(hipStream_t s, float* disp, uint32_t* disp_slots) {
+    k_disp_fold<<<1, 64, 0, s>>>(disp, disp_slots, 1);
 }
 
 void launch_rain(const DevParams& P, hipStream_t s, const int64_t* rain_in, const int64_t* dep,

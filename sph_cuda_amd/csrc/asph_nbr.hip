@@ -1,0 +1,724 @@
+// asph_nbr.hip - the two neighbour-loop kernels of the step on gfx950: lagged-Shepard density
+// fused with the EOS and warm-rain microphysics, and momentum fused with the theta update, the
+// second half kick, the sponge and the max|v| reduction.
+//
+// Work decomposition (mode A): one 512-thread workgroup per 8x8x8-cell brick of the Morton key
+// space (keys [b*512, b*512+512)). The workgroup owns the particles whose CURRENT cell lies in
+// its brick and stages, into LDS, every particle LISTED (by the last sort) in the brick's halo
+// (brick +- stencil cells), stored in row-major order of the halo cells with a per-cell start
+// table. For an owned particle i each stencil row (jz, jy) is then one contiguous LDS span, and
+// walking rows jz -> jy and spans left to right visits candidates exactly in the reference's
+// jz -> jy -> jx -> j order (asph_h100.cu:306-327, 444-495): the sums are bit-identical to a
+// thread-per-particle loop over the cell table.
+//
+// Candidate culling is exact: particles listed in cell c have moved at most delta cells (the
+// device-tracked maximum drift since the sort) so they lie within [c - delta, c + 1 + delta] on
+// each axis. Rows whose minimum metric distance exceeds the support, and cells of a row beyond
+// the row's chord, cannot hold an in-support particle and are skipped; surviving candidates go
+// through the reference test r2 < 4h^2 unchanged.
+//
+// The in-support pair work is separated from the candidate test by a per-thread queue of LDS
+// slots (QK entries): the test loop appends, and when any lane of the wavefront has a full queue
+// all lanes drain theirs in order. Lanes therefore spend the expensive pair body only on
+// in-support pairs, in the reference order.
+//
+// Mode B (drift since the last sort >= 0.95 cell, or a domain narrower than a halo) and the
+// overflow fallback of a brick (more than MAXP halo or MAXOWN owned particles) run the
+// thread-per-particle loop over the global cell table, with the same arithmetic.
+#include "asph_kernels.h"
+
+namespace asph {
+
+namespace {
+
+constexpr int NB = 9;              // Morton bits per brick: 8 x 8 x 8 cells
+constexpr int BS = 8;              // brick edge in cells
+constexpr int BT = 512;            // threads per workgroup (one per brick cell)
+constexpr int BW = BT / 64;        // wavefronts per workgroup
+constexpr int MAXP = 3072;         // halo particles held in LDS
+constexpr int MAXOWN = 768;        // owned particles per brick
+constexpr int QK = 8;              // queued in-support pairs per thread
+constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cells]
+
+template <int S> struct Halo {
+    static constexpr int H = BS + 2 * S;
+    static constexpr int HC = H * H * H;
+};
+
+__device__ __forceinline__ void min_image(const DevParams& P, float& rx, float& ry) {
+    if (P.flags & ASPH_F_MIN_IMAGE) {
+        if (rx > P.hLx) rx -= P.Lx; else if (rx < -P.hLx) rx += P.Lx;
+        if (ry > P.hLy) ry -= P.Ly; else if (ry < -P.hLy) ry += P.Ly;
+    }
+}
+
+__device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
+
+// Stencil z range clipped at the walls, asph_h100.cu:301-303.
+template <int S> __device__ __forceinline__ void zrange(const DevParams& P, int iz, int& z0, int& z1) {
+    bool interior = (iz >= S && iz < P.nz - S);
+    z0 = interior ? iz - S : max(iz - S, 0);
+    z1 = interior ? iz + S : min(iz + S, P.nz - 1);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Thread-per-particle loops over the global cell table (mode B and overflow fallback).
+// ---------------------------------------------------------------------------------------------
+template <int S>
+__device__ void dens_sum_global(const DevParams& P, float4 pi, const float4* __restrict__ pos,
+                                const float* __restrict__ rho_old, const uint32_t* __restrict__ cstart,
+                                float& S1, float& S2) {
+    int ix, iy, iz, z0, z1;
+    cell_trunc(P, pi.x, pi.y, pi.z, ix, iy, iz);
+    zrange<S>(P, iz, z0, z1);
+    S1 = 0.f;
+    S2 = 0.f;
+    for (int jz = z0; jz <= z1; ++jz) {
+        uint32_t kz = part1by2((uint32_t)jz) << 2;
+        for (int jy = iy - S; jy <= iy + S; ++jy) {
+            uint32_t kyz = kz | (part1by2((uint32_t)wrapi(jy, P.ny)) << 1);
+            for (int jx = ix - S; jx <= ix + S; ++jx) {
+                uint32_t k = kyz | part1by2((uint32_t)wrapi(jx, P.nx));
+                uint32_t s = cstart[k], e = cstart[k + 1];
+                for (uint32_t j = s; j < e; ++j) {
+                    float4 pj = pos[j];
+                    float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                    min_image(P, rx, ry);
+                    float dxm = rx * P.inv_dx, dym = ry * P.inv_dy, dzm = rz * P.inv_dz;
+                    float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+                    if (r2 < P.lim) {
+                        float W = fker(sqrtf(r2) * P.inv_h);
+                        S1 += W;
+                        S2 += W * (1.0f / fmaxf(rho_old[j], 1e-3f));
+                    }
+                }
+            }
+        }
+    }
+}
+
+struct MomAcc {
+    float ax, ay, az, avx, avy, avz;
+};
+
+// One in-support momentum pair (asph_h100.cu:460-491), shared by all paths.
+__device__ __forceinline__ void mom_pair(const DevParams& P, float rx, float ry, float rz,
+                                         float dxt, float dyt, float dzt, float r2t, float4 vi,
+                                         float rhoi, float inv_rhoi, float pri, float rhoj,
+                                         float4 vj, float prj, MomAcc& A) {
+    float q = sqrtf(r2t) * P.inv_h;
+    float kk = kgrad(P, q);
+    float gx = (kk * dxt) * P.inv_dx;
+    float gy = (kk * dyt) * P.inv_dy;
+    float gz = (kk * dzt) * P.inv_dz;
+    float inv_rhoj = 1.0f / rhoj;
+    float dp = (prj - pri) * (inv_rhoi * inv_rhoj);
+    float vdx = vi.x - vj.x, vdy = vi.y - vj.y, vdz = vi.z - vj.z;
+    float vdotr = __builtin_fmaf(vdz, rz, __builtin_fmaf(vdy, ry, vdx * rx));
+    float visc = 0.f;
+    if (vdotr < 0.f) {
+        float rp2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+        float rbar = 0.5f * (rhoi + rhoj);
+        visc = (P.visc_c * vdotr) / (((rp2 + P.eta2) * (rbar * rbar)) * rhoj);
+    }
+    float br = dp + visc;
+    A.ax = __builtin_fmaf(-br, gx, A.ax);
+    A.ay = __builtin_fmaf(-br, gy, A.ay);
+    A.az = __builtin_fmaf(-br, gz, A.az);
+    A.avx = __builtin_fmaf(-visc, gx, A.avx);
+    A.avy = __builtin_fmaf(-visc, gy, A.avy);
+    A.avz = __builtin_fmaf(-visc, gz, A.avz);
+}
+
+template <int S>
+__device__ void mom_sum_global(const DevParams& P, float4 pi, float4 vi, float rhoi, float pri,
+                               const float4* __restrict__ pos, const float4* __restrict__ vel,
+                               const float* __restrict__ rho, const float* __restrict__ prs,
+                               const uint32_t* __restrict__ cstart, MomAcc& A) {
+    int ix, iy, iz, z0, z1;
+    cell_trunc(P, pi.x, pi.y, pi.z, ix, iy, iz);
+    zrange<S>(P, iz, z0, z1);
+    const float inv_rhoi = 1.0f / rhoi;
+    for (int jz = z0; jz <= z1; ++jz) {
+        uint32_t kz = part1by2((uint32_t)jz) << 2;
+        for (int jy = iy - S; jy <= iy + S; ++jy) {
+            uint32_t kyz = kz | (part1by2((uint32_t)wrapi(jy, P.ny)) << 1);
+            for (int jx = ix - S; jx <= ix + S; ++jx) {
+                uint32_t k = kyz | part1by2((uint32_t)wrapi(jx, P.nx));
+                uint32_t s = cstart[k], e = cstart[k + 1];
+                for (uint32_t j = s; j < e; ++j) {
+                    float4 pj = pos[j];
+                    float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+                    min_image(P, rx, ry);
+                    float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
+                    float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
+                    // j == i has r2t = 0 and is excluded here as by `if (j==i) continue` (:452)
+                    if (r2t >= P.lim || r2t < 1e-12f) continue;
+                    mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri,
+                             fmaxf(rho[j], 1e-3f), vel[j], prs[j], A);
+                }
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Per-particle epilogues.
+// ---------------------------------------------------------------------------------------------
+// density ratio + clamp (:331-332), surface flux (:606-626), EOS + microphysics (:353-401),
+// Kessler accretion and rain evaporation. Writes rho_new, thq, prs and p' into vel.w (the
+// momentum kernel reads (v, p') of a neighbour with one load; the reference leaves vel.w unused).
+__device__ void dens_epilogue(const DevParams& P, const DensArgs& A, int i, float4 pi, float S1,
+                              float S2) {
+    const float z = pi.z;
+    const float ex = exner_ref(P, z);
+    const float pref = pres_of_exner(P, ex);
+    const float rref = pref / ((R_D * theta_ref(P, z)) * ex);
+    float rr = S2 > 1e-20f ? S1 / S2 : rref;
+    float rnew = fminf(fmaxf(rr, 0.5f * rref), 2.0f * rref);
+    A.rho_new[i] = rnew;
+    if (!(A.bits & 1)) return;
+
+    float4 t = A.thq[i];
+    if ((A.bits & 2) && (A.keys_sorted[i] & 0x24924924u) == 0u && z < P.dz) {
+        float Pi = asphm::fpow(fmaxf(pref, 1.0f) / P0, P.rd_cp);
+        float Ta = t.x * Pi;
+        t.x = t.x + (((P.c_h * P.v_bg) * (A.Ts - Ta)) * A.dt) / P.dz;
+        t.y = t.y + ((((P.beta_moist * P.c_e) * P.v_bg) * (A.qs_sfc - t.y)) * A.dt) / P.dz;
+        t.x = fminf(fmaxf(t.x, 100.f), P.theta_cap);
+        t.y = fminf(fmaxf(t.y, 0.f), P.qv_cap);
+    }
+    const float r = fmaxf(rnew, 1e-3f);
+    float dd = r - rref;
+    dd = fminf(fmaxf(dd, -0.3f * rref), 0.3f * rref);
+    const float ppr = P.cs2 * dd;
+    const float pfl = fmaxf(pref + ppr, 1.0f);
+    const float Pi = asphm::fpow(pfl / P0, P.rd_cp);
+    float th = t.x, qv = t.y, qc = t.z;
+    float T = th * Pi;
+    const float LvCp = L_V / CP;
+    float qs = qsat(T, pfl);
+    if (qv > qs || qc > 0.f) {
+        float dqc = 0.f;
+        for (int it = 0; it < 3; ++it) {
+            float Tn = T + LvCp * dqc;
+            float esn = esat(Tn);
+            float qsn = qsat(Tn, pfl);
+            float f = (qv - dqc) - qsn;
+            float fp = -1.f - dqsdT(Tn, esn, pfl) * LvCp;
+            dqc -= f / fp;
+            dqc = fminf(dqc, qv);
+            dqc = fmaxf(dqc, -qc);
+        }
+        qv -= dqc;
+        qc += dqc;
+        th += (L_V * dqc) / (CP * Pi);
+        T += LvCp * dqc;
+    }
+    int64_t dfx = 0;
+    if ((P.flags & ASPH_F_AUTOCONV) && qc > P.qc_auto) {
+        float dqr = (qc - P.qc_auto) * A.auto_f;
+        qc -= dqr;
+        dfx += to_fx(dqr);
+    }
+    int cid = -1;
+    if (P.flags & (ASPH_F_ACCRETION | ASPH_F_RAIN_EVAP)) {
+        cid = cell_of(P, pi.x, pi.y, pi.z);
+        float rf = rain_fx_to_float(A.rain[cid]);
+        float qr = (rf * P.mm) / r;
+        if ((P.flags & ASPH_F_ACCRETION) && qc > 0.f && qr > 0.f) {
+            float qcn = qc / (1.0f + (P.k_accr * A.dt) * asphm::fpow(qr, 0.875f));
+            dfx += to_fx(qc - qcn);
+            qc = qcn;
+        }
+        if ((P.flags & ASPH_F_RAIN_EVAP) && qr > 0.f) {
+            float qs2 = qsat(T, pfl);
+            if (qv < qs2) {
+                float rc = 0.001f * r;
+                float rq = rc * qr;
+                float C = 1.6f + 124.9f * asphm::fpow(rq, 0.2046f);
+                float E = (((1.0f - qv / qs2) * C) * asphm::fpow(rq, 0.525f)) /
+                          ((5.4e5f + 2.55e8f / (pfl * qs2)) * rc);
+                float dqe = fminf(fminf(E * A.dt, qr), qs2 - qv);
+                qv += dqe;
+                th -= (L_V * dqe) / (CP * Pi);
+                dfx -= to_fx(dqe);
+            }
+        }
+    }
+    if (dfx != 0) {
+        if (cid < 0) cid = cell_of(P, pi.x, pi.y, pi.z);
+        atomicAdd((unsigned long long*)&A.dep[cid], (unsigned long long)dfx);
+    }
+    th = fminf(fmaxf(th, 100.0f), P.theta_cap);
+    qv = fminf(fmaxf(qv, 0.0f), P.qv_cap);
+    qc = fmaxf(qc, 0.0f);
+    A.thq[i] = make_float4(th, qv, qc, 0.f);
+    A.prs[i] = ppr;
+    reinterpret_cast<float*>(A.vel)[4 * (size_t)i + 3] = ppr;
+}
+
+// momentum finish (:497-510) + theta update, second kick, sponge (mode 1). Returns |v| maxima.
+__device__ void mom_epilogue(const DevParams& P, const MomArgs& A, int i, float4 pi, float4 vi,
+                             float4 ti, float pri, MomAcc M, float& mx, float& my, float& mz) {
+    float ax = M.ax * P.mm, ay = M.ay * P.mm, az = M.az * P.mm;
+    float avx = M.avx * P.mm, avy = M.avy * P.mm, avz = M.avz * P.mm;
+    const float z = pi.z;
+    const float ex = exner_ref(P, z);
+    const float thr = theta_ref(P, z);
+    float thv = ti.x * (1.0f + 0.608f * ti.y);
+    float thvr = thr * (1.0f + 0.608f * qv_ref(P, z));
+    az = az + GRAV * ((thv - thvr) / thvr - ti.z);
+    A.acc[i] = make_float4(ax, ay, az, 0.f);
+    float Pi = asphm::fpow(fmaxf(pres_of_exner(P, ex) + pri, 1.0f) / P0, P.rd_cp);
+    float dth = -__builtin_fmaf(vi.z, avz, __builtin_fmaf(vi.y, avy, vi.x * avx)) / (CP * Pi);
+    dth = fminf(fmaxf(dth, -P.dthdt_cap), P.dthdt_cap);
+    if (A.mode == 0) {
+        A.dthdt[i] = dth;
+        return;
+    }
+    float4 t = ti;
+    t.x = fminf(fmaxf(__builtin_fmaf(dth, A.dt, t.x), 100.0f), P.theta_cap);
+    A.thq[i] = t;
+    const float hdt = 0.5f * A.dt;
+    float4 v = vi;
+    v.x = __builtin_fmaf(ax, hdt, v.x);
+    v.y = __builtin_fmaf(ay, hdt, v.y);
+    v.z = __builtin_fmaf(az, hdt, v.z);
+    if ((P.flags & ASPH_F_SPONGE) && z >= P.sponge_zs) {
+        float tt = (z - P.sponge_zs) / P.sponge_depth;
+        float sg = (P.sponge_sigma * tt) * tt;
+        v.z *= 1.0f / (1.0f + sg * A.dt);
+    }
+    v.w = 0.f;
+    A.vel_out[i] = v;
+    mx = fmaxf(mx, fabsf(v.x));
+    my = fmaxf(my, fabsf(v.y));
+    mz = fmaxf(mz, fabsf(v.z));
+}
+
+// ---------------------------------------------------------------------------------------------
+// Brick staging shared by both kernels.
+// ---------------------------------------------------------------------------------------------
+template <int S, bool MOM> struct Smem {
+    float4 p[MAXP];                          // x, y, z, and 1/max(rho_old,1e-3) or max(rho,1e-3)
+    uint32_t j[MOM ? MAXP : 1];              // global index of each staged particle (momentum)
+    uint16_t off[Halo<S>::HC + 1];           // exclusive start of each halo cell, row-major
+    uint32_t own[MAXOWN];                    // owned particles (global indices)
+    uint16_t q[QK][BT];                      // per-thread queue of LDS slots
+    uint32_t wsum[BW];
+    int nown;
+    int total;
+    int modeA;
+    float delta[3];
+};
+
+struct BrickGeo {
+    int bx0, by0, bz0;
+    bool wraps;  // the halo crosses a periodic face
+};
+
+__device__ __forceinline__ BrickGeo brick_geo(const DevParams& P, int S) {
+    BrickGeo g;
+    uint32_t kb = (uint32_t)blockIdx.x << NB;
+    g.bx0 = (int)compact1by2(kb);
+    g.by0 = (int)compact1by2(kb >> 1);
+    g.bz0 = (int)compact1by2(kb >> 2);
+    const int H = BS + 2 * S;
+    g.wraps = (g.bx0 - S < 0) || (g.bx0 - S + H > P.nx) || (g.by0 - S < 0) || (g.by0 - S + H > P.ny);
+    return g;
+}
+
+// Reads the drift bound since the last sort (cells) and decides between mode A and mode B.
+template <int S, bool MOM>
+__device__ void prologue(const DevParams& P, const float* disp, const uint32_t* disp_slots,
+                         Smem<S, MOM>& sm) {
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        float m[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            float v = __uint_as_float(disp_slots[a * 64 + tid]);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+            m[a] = v;
+        }
+        if (tid == 0) {
+            bool ok = P.nx >= Halo<S>::H && P.ny >= Halo<S>::H;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                float d = disp[a] + m[a];
+                sm.delta[a] = d + DELTA_EPS;
+                ok = ok && (d < 0.95f);
+            }
+            sm.modeA = ok ? 1 : 0;
+            sm.nown = 0;
+        }
+    }
+    __syncthreads();
+}
+
+// Stages the halo of this workgroup's brick; returns false if it does not fit.
+template <int S, bool MOM, class Rec>
+__device__ bool stage(const DevParams& P, const BrickGeo& g, const float4* __restrict__ pos,
+                      const uint32_t* __restrict__ cstart, Smem<S, MOM>& sm, Rec rec) {
+    constexpr int H = Halo<S>::H;
+    constexpr int HC = Halo<S>::HC;
+    constexpr int PER = (HC + BT - 1) / BT;
+    const int tid = threadIdx.x;
+    const int hx0 = g.bx0 - S, hy0 = g.by0 - S, hz0 = g.bz0 - S;
+    // counts of PER consecutive halo cells per thread, then a block-wide exclusive scan
+    uint32_t cnt[PER];
+    uint32_t start[PER];
+    uint32_t local = 0;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        int c = tid * PER + k;
+        cnt[k] = 0;
+        start[k] = 0;
+        if (c < HC) {
+            int hx = c % H, hy = (c / H) % H, hz = c / (H * H);
+            int gz = hz0 + hz;
+            if (gz >= 0 && gz < P.nz) {
+                uint32_t key = morton3((uint32_t)wrapi(hx0 + hx, P.nx), (uint32_t)wrapi(hy0 + hy, P.ny),
+                                       (uint32_t)gz);
+                uint32_t s = cstart[key];
+                start[k] = s;
+                cnt[k] = cstart[key + 1] - s;
+            }
+        }
+        local += cnt[k];
+    }
+    const int lane = tid & 63, w = tid >> 6;
+    uint32_t x = local;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        uint32_t y = __shfl_up(x, o);
+        if (lane >= o) x += y;
+    }
+    if (lane == 63) sm.wsum[w] = x;
+    __syncthreads();
+    uint32_t woff = 0, total = 0;
+#pragma unroll
+    for (int k = 0; k < BW; ++k) {
+        uint32_t s = sm.wsum[k];
+        if (k < w) woff += s;
+        total += s;
+    }
+    const bool fits = total <= (uint32_t)MAXP;
+    uint32_t run = woff + x - local;
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        int c = tid * PER + k;
+        if (c < HC) sm.off[c] = (uint16_t)(fits ? run : 0);
+        if (fits && c < HC) {
+            for (uint32_t m = 0; m < cnt[k]; ++m) {
+                uint32_t jg = start[k] + m;
+                float4 pj = pos[jg];
+                rec(sm, run + m, jg, pj);
+                int ix, iy, iz;
+                cell_trunc(P, pj.x, pj.y, pj.z, ix, iy, iz);
+                if ((unsigned)(ix - g.bx0) < (unsigned)BS && (unsigned)(iy - g.by0) < (unsigned)BS &&
+                    (unsigned)(iz - g.bz0) < (unsigned)BS) {
+                    // owned particles are listed within one cell of the brick (drift < 1 cell);
+                    // counting them only there keeps each one once
+                    int hx = c % H, hy = (c / H) % H, hz = c / (H * H);
+                    if (hx >= S - 1 && hx <= S + BS && hy >= S - 1 && hy <= S + BS && hz >= S - 1 &&
+                        hz <= S + BS) {
+                        int idx = atomicAdd(&sm.nown, 1);
+                        if (idx < MAXOWN) sm.own[idx] = jg;
+                    }
+                }
+            }
+        }
+        run += cnt[k];
+    }
+    if (tid == 0) {
+        sm.off[HC] = (uint16_t)(fits ? total : 0);
+        sm.total = (int)total;
+    }
+    __syncthreads();
+    return fits && sm.nown <= MAXOWN;
+}
+
+// Calls fn(j) for every particle whose current cell is in the brick, found among the particles
+// listed within one cell of it (overflow fallback; thread-strided).
+template <int S, class Fn>
+__device__ void for_each_owned_global(const DevParams& P, const BrickGeo& g, const float4* __restrict__ pos,
+                                      const uint32_t* __restrict__ cstart, Fn fn) {
+    constexpr int W = BS + 2;
+    for (int c = threadIdx.x; c < W * W * W; c += BT) {
+        int ux = g.bx0 - 1 + c % W, uy = g.by0 - 1 + (c / W) % W, gz = g.bz0 - 1 + c / (W * W);
+        if (gz < 0 || gz >= P.nz) continue;
+        uint32_t key = morton3((uint32_t)wrapi(ux, P.nx), (uint32_t)wrapi(uy, P.ny), (uint32_t)gz);
+        for (uint32_t jg = cstart[key]; jg < cstart[key + 1]; ++jg) {
+            float4 pj = pos[jg];
+            int ix, iy, iz;
+            cell_trunc(P, pj.x, pj.y, pj.z, ix, iy, iz);
+            if ((unsigned)(ix - g.bx0) < (unsigned)BS && (unsigned)(iy - g.by0) < (unsigned)BS &&
+                (unsigned)(iz - g.bz0) < (unsigned)BS)
+                fn((int)jg, pj);
+        }
+    }
+}
+
+// Walks the culled stencil rows of an owned particle: calls cand(slot) for every candidate in
+// the reference order.
+template <int S, class Cand>
+__device__ __forceinline__ void walk_rows(const DevParams& P, const BrickGeo& g, const float* delta,
+                                          const uint16_t* off, float4 pi, Cand cand) {
+    constexpr int H = Halo<S>::H;
+    int ix, iy, iz, z0, z1;
+    cell_trunc(P, pi.x, pi.y, pi.z, ix, iy, iz);
+    zrange<S>(P, iz, z0, z1);
+    const float xi = pi.x * P.inv_dx, yi = pi.y * P.inv_dy, zi = pi.z * P.inv_dz;
+    const float limc = P.lim * 1.0001f;
+    const int hx0 = g.bx0 - S, hy0 = g.by0 - S, hz0 = g.bz0 - S;
+    for (int jz = z0; jz <= z1; ++jz) {
+        float ez = fmaxf(0.f, fmaxf(((float)jz - delta[2]) - zi, zi - ((float)(jz + 1) + delta[2])));
+        float ez2 = ez * ez;
+        const int rowz = (jz - hz0) * H;
+        for (int jy = iy - S; jy <= iy + S; ++jy) {
+            float ey = fmaxf(0.f, fmaxf(((float)jy - delta[1]) - yi, yi - ((float)(jy + 1) + delta[1])));
+            float d2 = __builtin_fmaf(ey, ey, ez2);
+            if (d2 >= limc) continue;
+            float ext = sqrtf(limc - d2) + DELTA_EPS;
+            int xlo = max(ix - S, (int)ceilf(xi - ext - 1.0f - delta[0]));
+            int xhi = min(ix + S, (int)floorf(xi + ext + delta[0]));
+            if (xlo > xhi) continue;
+            const int row = (rowz + (jy - hy0)) * H - hx0;
+            const int a = off[row + xlo], e = off[row + xhi + 1];
+            for (int p = a; p < e; ++p) cand(p);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Density + EOS + microphysics.
+// ---------------------------------------------------------------------------------------------
+template <int S, bool WRAP>
+__device__ void dens_own(const DevParams& P, const DensArgs& A, const BrickGeo& g, Smem<S, false>& sm,
+                         int i, bool active) {
+    const int tid = threadIdx.x;
+    float4 pi = active ? A.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+    float S1 = 0.f, S2 = 0.f;
+    int qn = 0;
+    auto diff = [&](float4 pj, float& dxm, float& dym, float& dzm) {
+        float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+        if (WRAP) min_image(P, rx, ry);
+        dxm = rx * P.inv_dx; dym = ry * P.inv_dy; dzm = rz * P.inv_dz;
+    };
+    auto flush = [&]() {
+        for (int k = 0; wave_any(k < qn); ++k) {
+            if (k < qn) {
+                float4 pj = sm.p[sm.q[k][tid]];
+                float dxm, dym, dzm;
+                diff(pj, dxm, dym, dzm);
+                float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+                float W = fker(sqrtf(r2) * P.inv_h);
+                S1 += W;
+                S2 += W * pj.w;
+            }
+        }
+        qn = 0;
+    };
+    if (active) {
+        walk_rows<S>(P, g, sm.delta, sm.off, pi, [&](int p) {
+            float4 pj = sm.p[p];
+            float dxm, dym, dzm;
+            diff(pj, dxm, dym, dzm);
+            float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+            if (r2 < P.lim) {
+                sm.q[qn][tid] = (uint16_t)p;
+                ++qn;
+            }
+            if (wave_any(qn == QK)) flush();
+        });
+    }
+    flush();
+    if (active) dens_epilogue(P, A, i, pi, S1, S2);
+}
+
+template <int S>
+__global__ __launch_bounds__(BT) void k_density_eos_brick(DevParams P, DensArgs A) {
+    __shared__ Smem<S, false> sm;
+    const int tid = threadIdx.x;
+    if (blockIdx.x == 0 && tid < 3 * 64) A.maxv[tid] = 0u;
+    prologue<S, false>(P, A.disp, A.disp_slots, sm);
+    if (!sm.modeA) {  // mode B: particle per thread by index
+        int i = blockIdx.x * BT + tid;
+        if (i < P.N) {
+            float4 pi = A.pos[i];
+            float S1, S2;
+            dens_sum_global<S>(P, pi, A.pos, A.rho_old, A.cstart, S1, S2);
+            dens_epilogue(P, A, i, pi, S1, S2);
+        }
+        return;
+    }
+    const BrickGeo g = brick_geo(P, S);
+    const float* rho_old = A.rho_old;
+    bool ok = stage<S, false>(P, g, A.pos, A.cstart, sm, [&](Smem<S, false>& s, uint32_t slot, uint32_t jg, float4 pj) {
+        s.p[slot] = make_float4(pj.x, pj.y, pj.z, 1.0f / fmaxf(rho_old[jg], 1e-3f));
+    });
+    if (!ok) {
+        for_each_owned_global<S>(P, g, A.pos, A.cstart, [&](int i, float4 pi) {
+            float S1, S2;
+            dens_sum_global<S>(P, pi, A.pos, A.rho_old, A.cstart, S1, S2);
+            dens_epilogue(P, A, i, pi, S1, S2);
+        });
+        return;
+    }
+    const int nown = sm.nown;
+    for (int r = 0; r < nown; r += BT) {
+        int k = r + tid;
+        bool active = k < nown;
+        int i = active ? (int)sm.own[k] : 0;
+        if (g.wraps) dens_own<S, true>(P, A, g, sm, i, active);
+        else dens_own<S, false>(P, A, g, sm, i, active);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+// Momentum + theta update + kick + sponge + max|v|.
+// ---------------------------------------------------------------------------------------------
+template <int S, bool WRAP>
+__device__ void mom_own(const DevParams& P, const MomArgs& A, const BrickGeo& g, Smem<S, true>& sm,
+                        int i, bool active, float& mx, float& my, float& mz) {
+    const int tid = threadIdx.x;
+    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi, ti = pi;
+    float rhoi = 1.f, pri = 0.f;
+    if (active) {
+        pi = A.pos[i];
+        vi = A.vel[i];
+        ti = A.thq[i];
+        rhoi = fmaxf(A.rho[i], 1e-3f);
+        pri = A.prs[i];
+    }
+    const float inv_rhoi = 1.0f / rhoi;
+    MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int qn = 0;
+    auto diff = [&](float4 pj, float& rx, float& ry, float& rz) {
+        rx = pi.x - pj.x; ry = pi.y - pj.y; rz = pi.z - pj.z;
+        if (WRAP) min_image(P, rx, ry);
+    };
+    auto flush = [&]() {
+        for (int k = 0; wave_any(k < qn); ++k) {
+            if (k < qn) {
+                int p = sm.q[k][tid];
+                float4 pj = sm.p[p];
+                float4 vj = A.vel[sm.j[p]];   // (vx, vy, vz, p'_j)
+                float rx, ry, rz;
+                diff(pj, rx, ry, rz);
+                float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
+                float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
+                mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, pj.w, vj, vj.w, M);
+            }
+        }
+        qn = 0;
+    };
+    if (active) {
+        walk_rows<S>(P, g, sm.delta, sm.off, pi, [&](int p) {
+            float4 pj = sm.p[p];
+            float rx, ry, rz;
+            diff(pj, rx, ry, rz);
+            float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
+            float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
+            if (r2t < P.lim && r2t >= 1e-12f) {
+                sm.q[qn][tid] = (uint16_t)p;
+                ++qn;
+            }
+            if (wave_any(qn == QK)) flush();
+        });
+    }
+    flush();
+    if (active) mom_epilogue(P, A, i, pi, vi, ti, pri, M, mx, my, mz);
+}
+
+__device__ __forceinline__ void reduce_maxv(uint32_t* maxv, float mx, float my, float mz) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, o));
+        my = fmaxf(my, __shfl_xor(my, o));
+        mz = fmaxf(mz, __shfl_xor(mz, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        int slot = (blockIdx.x * BW + (threadIdx.x >> 6)) & 63;
+        atomicMax(&maxv[slot], __float_as_uint(mx));
+        atomicMax(&maxv[64 + slot], __float_as_uint(my));
+        atomicMax(&maxv[128 + slot], __float_as_uint(mz));
+    }
+}
+
+template <int S>
+__global__ __launch_bounds__(BT) void k_momentum_kdk_brick(DevParams P, MomArgs A) {
+    __shared__ Smem<S, true> sm;
+    const int tid = threadIdx.x;
+    float mx = 0.f, my = 0.f, mz = 0.f;
+    prologue<S, true>(P, A.disp, A.disp_slots, sm);
+    if (!sm.modeA) {
+        int i = blockIdx.x * BT + tid;
+        if (i < P.N) {
+            float4 pi = A.pos[i], vi = A.vel[i], ti = A.thq[i];
+            float rhoi = fmaxf(A.rho[i], 1e-3f), pri = A.prs[i];
+            MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            mom_sum_global<S>(P, pi, vi, rhoi, pri, A.pos, A.vel, A.rho, A.prs, A.cstart, M);
+            mom_epilogue(P, A, i, pi, vi, ti, pri, M, mx, my, mz);
+        }
+        if (A.mode) reduce_maxv(A.maxv, mx, my, mz);
+        return;
+    }
+    const BrickGeo g = brick_geo(P, S);
+    const float* rho = A.rho;
+    bool ok = stage<S, true>(P, g, A.pos, A.cstart, sm, [&](Smem<S, true>& s, uint32_t slot, uint32_t jg, float4 pj) {
+        s.p[slot] = make_float4(pj.x, pj.y, pj.z, fmaxf(rho[jg], 1e-3f));
+        s.j[slot] = jg;
+    });
+    if (!ok) {
+        for_each_owned_global<S>(P, g, A.pos, A.cstart, [&](int i, float4 pi) {
+            float4 vi = A.vel[i], ti = A.thq[i];
+            float rhoi = fmaxf(A.rho[i], 1e-3f), pri = A.prs[i];
+            MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+            mom_sum_global<S>(P, pi, vi, rhoi, pri, A.pos, A.vel, A.rho, A.prs, A.cstart, M);
+            mom_epilogue(P, A, i, pi, vi, ti, pri, M, mx, my, mz);
+        });
+    } else {
+        const int nown = sm.nown;
+        for (int r = 0; r < nown; r += BT) {
+            int k = r + tid;
+            bool active = k < nown;
+            int i = active ? (int)sm.own[k] : 0;
+            if (g.wraps) mom_own<S, true>(P, A, g, sm, i, active, mx, my, mz);
+            else mom_own<S, false>(P, A, g, sm, i, active, mx, my, mz);
+        }
+    }
+    if (A.mode) reduce_maxv(A.maxv, mx, my, mz);
+}
+
+}  // namespace
+
+int brick_count(const DevParams& P) {
+    uint32_t nkeys = 1u << P.end_bit;
+    return (int)std::max<uint32_t>(1u, nkeys >> NB);
+}
+
+void launch_density_eos(const DevParams& P, hipStream_t s, const DensArgs& A) {
+    dim3 g(brick_count(P));
+    switch (P.S) {
+        case 1: k_density_eos_brick<1><<<g, BT, 0, s>>>(P, A); break;
+        case 2: k_density_eos_brick<2><<<g, BT, 0, s>>>(P, A); break;
+        case 3: k_density_eos_brick<3><<<g, BT, 0, s>>>(P, A); break;
+        default: k_density_eos_brick<4><<<g, BT, 0, s>>>(P, A); break;
+    }
+}
+
+void launch_momentum_kdk(const DevParams& P, hipStream_t s, const MomArgs& A) {
+    dim3 g(brick_count(P));
+    switch (P.S) {
+        case 1: k_momentum_kdk_brick<1><<<g, BT, 0, s>>>(P, A); break;
+        case 2: k_momentum_kdk_brick<2><<<g, BT, 0, s>>>(P, A); break;
+        case 3: k_momentum_kdk_brick<3><<<g, BT, 0, s>>>(P, A); break;
+        default: k_momentum_kdk_brick<4><<<g, BT, 0, s>>>(P, A); break;
+    }
+}
+
+}  // namespace asph

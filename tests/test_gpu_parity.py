@@ -17,7 +17,13 @@ TOL = 1e-5
 KINDS = ["ref", "dry", "moist"]
 
 
-def _pair(kind, n=(12, 12, 10), shuffle=False, seed=0):
+# (16, 16, 12) is wide enough for the LDS brick path (halo 14 cells at stencil 3); (12, 12, 10)
+# exercises the thread-per-particle path of the same kernels.
+N_BRICK = (16, 16, 12)
+N_SMALL = (12, 12, 10)
+
+
+def _pair(kind, n=N_BRICK, shuffle=False, seed=0):
     cfg, ic, dt = small_config(kind, n)
     st = s.ic_generate(cfg, ic)
     if shuffle:
@@ -67,8 +73,9 @@ def test_sort_ties_and_multi_tile():
 
 
 @pytest.mark.parametrize("kind", KINDS)
-def test_force_parity(kind):
-    cfg, dt, st, g, o = _pair(kind)
+@pytest.mark.parametrize("n", [N_BRICK, N_SMALL])
+def test_force_parity(kind, n):
+    cfg, dt, st, g, o = _pair(kind, n=n)
     g.sort(); o.sort()
     g.force(dt); o.force(dt)
     bad, errs = state_close(g.get_state(), o.get_state(), TOL)
@@ -83,8 +90,9 @@ def test_force_parity(kind):
 
 
 @pytest.mark.parametrize("kind", KINDS)
-def test_stage_parity(kind):
-    cfg, dt, st, g, o = _pair(kind)
+@pytest.mark.parametrize("n", [N_BRICK, N_SMALL])
+def test_stage_parity(kind, n):
+    cfg, dt, st, g, o = _pair(kind, n=n)
     for x in (g, o):
         x.sort()
         x.force(dt)
@@ -103,7 +111,7 @@ def test_stage_parity(kind):
 @pytest.mark.parametrize("kind", KINDS)
 def test_run_parity(kind):
     """The reference loop (sort gating, CFL controller) for 12 steps, state after each call."""
-    cfg, dt, st, g, o = _pair(kind, n=(14, 13, 12))
+    cfg, dt, st, g, o = _pair(kind, n=(17, 16, 13))
     for k in range(3):
         rc_g, sg = g.run(max_steps=4, dt=dt if k == 0 else 0.0)
         rc_o, so = o.run(max_steps=4, dt=dt if k == 0 else 0.0)
@@ -180,3 +188,50 @@ def test_cfl_trip_and_abort(strict):
         assert np.sqrt((v.astype(np.float64) ** 2).sum(1)).max() <= 0.5 * cfg.cs + 5.0
         bad, errs = state_close(g.get_state(), o.get_state(), TOL)
         assert not bad, errs
+
+
+def test_clumped_overflow_fallback():
+    """A third of the particles collapsed onto other cells: bricks whose halo exceeds the LDS
+    capacity take the global-memory fallback; results are unchanged."""
+    cfg, ic, dt = small_config("moist", (24, 24, 16))
+    st = s.ic_generate(cfg, ic)
+    rng = np.random.default_rng(11)
+    src = rng.choice(cfg.n, cfg.n // 3, replace=False)
+    dst = rng.integers(0, 64, src.size)          # onto 64 particles in one corner region
+    st.pos[src, :3] = st.pos[dst, :3] + rng.uniform(-1, 1, (src.size, 3)).astype(np.float32)
+    st.pos[:, 2] = np.clip(st.pos[:, 2], 1.0, cfg.nz * cfg.dz - 1.0)
+    st.pos[:, 0] = np.mod(st.pos[:, 0], cfg.nx * cfg.dx)
+    st.pos[:, 1] = np.mod(st.pos[:, 1], cfg.ny * cfg.dy)
+    g, o = s.Sim(cfg), Oracle(cfg)
+    g.set_state(st)
+    o.set_state(st)
+    for x in (g, o):
+        x.sort()
+        x.force(dt)
+    bad, errs = state_close(g.get_state(), o.get_state(), TOL)
+    assert not bad, errs
+    ga, _, _ = g.get_aux()
+    oa, _, _ = o.get_aux()
+    for c in range(3):
+        assert relnorm(ga[:, c], oa[:, c]) <= TOL
+
+
+def test_large_drift_without_sort():
+    """Steps without re-sorting while particles drift more than one cell: the kernels switch to
+    the thread-per-particle path (drift bound >= 0.95 cell) and still match the oracle, which
+    iterates the stale cell table exactly as the reference does."""
+    cfg, ic, dt = small_config("dry", N_BRICK)
+    st = s.ic_generate(cfg, ic)
+    rng = np.random.default_rng(5)
+    st.vel[:, :3] = rng.normal(0, 40.0, (cfg.n, 3)).astype(np.float32)   # ~0.1 cell per step
+    g, o = s.Sim(cfg), Oracle(cfg)
+    g.set_state(st)
+    o.set_state(st)
+    for x in (g, o):
+        x.sort()
+        x.force(dt)
+    for k in range(14):
+        for x in (g, o):
+            x.step(dt, k * dt, do_sort=False)
+        bad, errs = state_close(g.get_state(), o.get_state(), TOL)
+        assert not bad, (k, errs)
