@@ -241,6 +241,25 @@ int asph_run(asph_t* h, const asph_run_args* args, asph_run_stats* stats);
 /* Simulated time and sort-gating clock of the handle. */
 int asph_get_clock(const asph_t* h, double* simt, double* since_sort, float* sort_dt);
 
+/* --- ensembles and graph replay ------------------------------------------------------------ */
+
+/* A handle holding n_members independent domains (no reference counterpart: the reference runs
+ * one domain per process). Members are concatenated in member order in every state array
+ * (asph_member_range gives each member's slice); all members share the stencil (ceil(2h)) and
+ * the ASPH_F_GATHER_ACC choice. Every stage runs as one launch over all members. */
+int asph_ensemble_create(const asph_config* cfgs, int32_t n_members, void* stream, asph_t** out);
+int32_t asph_member_count(const asph_t* h);
+int asph_member_range(const asph_t* h, int32_t m, int64_t* first, int64_t* count);
+/* Per-axis max |v| of member m after the last momentum stage (synchronises). */
+int asph_member_max_vel(asph_t* h, int32_t m, float out[3]);
+
+/* nsteps KDK steps at a fixed dt replayed from captured hipGraphs (one graph per buffer
+ * assignment and sort flag, captured on first use), with no host synchronisation between
+ * steps; the simulated time advances on the device. The sort runs every sort_every steps
+ * (every step with ASPH_F_SORT_EVERY_STEP; sort_every <= 0 gates it like asph_run, with the
+ * interval of the last readback). Single domains and ensembles alike. */
+int asph_run_graph(asph_t* h, int64_t nsteps, float dt, int32_t sort_every, asph_run_stats* stats);
+
 /* --- dumps -------------------------------------------------------------------------------- */
 
 /* Coarse block means of [theta,qv,qc,u,v,w] (kernel_coarse_dump, asph_h100.cu:680-703) into

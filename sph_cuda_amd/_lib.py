@@ -135,6 +135,11 @@ SIGNATURES = {
     "asph_column_diag": (C.c_int, [P, FP]),
     "asph_write_dump": (C.c_int, [P, C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                   C.c_float, C.c_char_p, C.c_size_t]),
+    "asph_ensemble_create": (C.c_int, [C.POINTER(Config), C.c_int32, P, C.POINTER(P)]),
+    "asph_member_count": (C.c_int32, [P]),
+    "asph_member_range": (C.c_int, [P, C.c_int32, I64P, I64P]),
+    "asph_member_max_vel": (C.c_int, [P, C.c_int32, FP]),
+    "asph_run_graph": (C.c_int, [P, C.c_int64, C.c_float, C.c_int32, C.POINTER(RunStats)]),
     "asph_set_profiling": (C.c_int, [P, C.c_int32]),
     "asph_kernel_times": (C.c_int, [P, DP, I64P]),
     "asph_synchronize": (C.c_int, [P]),

@@ -162,4 +162,39 @@ ASPH_DH int64_t to_fx(float v) {
 // Host: fill DevParams from a configuration (float evaluation, the order matches the oracle).
 void make_params(const asph_config& c, DevParams& P);
 
+// One domain of a handle. A plain run is a handle with one member; an ensemble concatenates its
+// members' particles (and cells, one per particle) and columns, and prefixes their Morton keys
+// with the member index so that one sort keeps every member contiguous and cell-ordered.
+struct MemberDesc {
+    DevParams P;
+    int32_t p0;     // first particle (= first cell) of the member in the concatenated arrays
+    int32_t col0;   // first surface column
+    uint32_t k0;    // key offset: member index << key bits
+    int32_t first_block;
+};
+
+// Work-item map of one launch: the member and local start of a block of 256 particles / cells,
+// or of an 8x8x8-cell brick (base = local Morton key base, ord = brick ordinal in the member).
+struct BlockMap {
+    int32_t m;
+    int32_t base;
+    int32_t ord;
+    int32_t pad;
+};
+
+// Surface skin temperature and saturation humidity at simulated time t (asph_h100.cu:989-992).
+ASPH_DH void sfc_params(const DevParams& P, double t, float& Ts, float& qs) {
+    float r = fminf(fmaxf((float)t / P.ramp, 0.f), 1.f);
+    float sw = (r * r) * (3.f - 2.f * r);
+    Ts = P.th0 + P.dts_skin * sw;
+    qs = qsat(Ts, pres_ref(P, 0.0f));
+}
+// 1 - exp(-k dt) of the autoconversion (asph_h100.cu:390).
+ASPH_DH float auto_factor(const DevParams& P, float dt) { return 1.0f - asphm::fexp(-P.rate_auto * dt); }
+// Fraction of a cell's rain leaving through its floor per step (asph_h100.cu:587-588).
+ASPH_DH float rain_frac(const DevParams& P, float dt) {
+    float f = (P.v_rain * dt) * P.inv_dz;
+    return f > 1.f ? 1.f : f;
+}
+
 }  // namespace asph

@@ -1,9 +1,20 @@
-// asph_kernels.h - host-side launchers of the gfx950 kernels (asph_kernels.hip, asph_sort.hip).
+// asph_kernels.h - host-side launchers of the gfx950 kernels (asph_kernels.hip, asph_sort.hip,
+// asph_nbr.hip).
 #pragma once
 
 #include "asph_internal.h"
 
 namespace asph {
+
+// Device tables describing one handle's members and the work maps of its launches.
+struct Launch {
+    const MemberDesc* mem;   // [n_members]
+    const BlockMap* pmap;    // [npblk]: blocks of 256 particles (= cells) of one member each
+    const BlockMap* bmap;    // [nbblk]: 8x8x8-cell bricks of one member each
+    int npblk;
+    int nbblk;
+    int n_members;
+};
 
 struct DensArgs {
     const float4* pos;
@@ -16,11 +27,13 @@ struct DensArgs {
     const uint32_t* keys_sorted;
     const int64_t* rain;
     int64_t* dep;
-    uint32_t* maxv;
+    uint32_t* maxv;              // [n_members][3][64], cleared here for the momentum kernel
     const float* disp;
     const uint32_t* disp_slots;
-    float dt, auto_f, Ts, qs_sfc;
-    int bits;                    // bit 0: EOS + microphysics, bit 1: surface flux
+    const double* clock;         // simulated time at the start of the step (device), or null:
+    double simt;                 // then this value is used
+    float dt;
+    int bits;                    // bit 0: EOS + microphysics, bit 1: surface flux (if flagged)
 };
 
 struct MomArgs {
@@ -33,42 +46,45 @@ struct MomArgs {
     float4* acc;
     float* dthdt;
     float4* vel_out;
-    uint32_t* maxv;
+    uint32_t* maxv;              // [n_members][3][64]
     const float* disp;
     const uint32_t* disp_slots;
     float dt;
     int mode;                    // 0: force evaluation, 1: step
 };
 
-void launch_kick_drift(const DevParams& P, hipStream_t s, float4* pos, float4* vel,
-                       const float4* acc, int64_t* dep, float* disp, uint32_t* disp_slots, float dt);
-void launch_disp_reset(hipStream_t s, float* disp, uint32_t* disp_slots);
-void launch_density_eos(const DevParams& P, hipStream_t s, const DensArgs& A);
-void launch_momentum_kdk(const DevParams& P, hipStream_t s, const MomArgs& A);
-int brick_count(const DevParams& P);
-void launch_rain(const DevParams& P, hipStream_t s, const int64_t* rain_in, const int64_t* dep,
-                 int64_t* rain_out, int64_t* precip, unsigned long long* clipped, float frac,
-                 int sediment);
+int particle_block_size();
+void launch_step_begin(hipStream_t s, float* disp, uint32_t* disp_slots, int reset, double* clock,
+                       int clock_mode, double t, float dt);
+void launch_kick_drift(const Launch& L, hipStream_t s, float4* pos, float4* vel, const float4* acc,
+                       int64_t* dep, uint32_t* disp_slots, float dt);
+void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A);
+void launch_momentum_kdk(const Launch& L, int S, hipStream_t s, const MomArgs& A);
+int brick_bits();
+void launch_rain(const Launch& L, hipStream_t s, const int64_t* rain_in, const int64_t* dep,
+                 int64_t* rain_out, int64_t* precip, unsigned long long* clipped, float dt);
 void launch_rain_merge(hipStream_t s, int n, int64_t* rain, int64_t* dep);
+void launch_keys(const Launch& L, hipStream_t s, const float4* pos, uint32_t* keys);
 void launch_gather(hipStream_t s, int N, const uint32_t* perm, const float4* pA, float4* pB,
                    const float4* vA, float4* vB, const float4* tA, float4* tB, const float* rA,
                    float* rB, const float4* aA, float4* aB);
 void launch_cell_start(hipStream_t s, int N, uint32_t nkeys, const uint32_t* keys, uint32_t* cstart);
-void launch_clamp_vel(hipStream_t s, int N, float4* vel, float vmax);
-void launch_audit(const DevParams& P, hipStream_t s, const float4* pos, const float4* vel,
+void launch_clamp_vel(const Launch& L, hipStream_t s, float4* vel, float vmax, float cs_frac);
+void launch_audit(const Launch& L, hipStream_t s, const float4* pos, const float4* vel,
                   const float4* thq, const int64_t* rain, const int64_t* dep, const int64_t* precip,
                   double* tot);
-void launch_coarse_dump(const DevParams& P, hipStream_t s, const uint32_t* cstart, const float4* vel,
-                        const float4* thq, float* out, int sx, int sy, int sz, int cx, int cy, int cz);
-void launch_column_diag(const DevParams& P, hipStream_t s, const uint32_t* cstart, const float4* thq,
-                        const float4* vel, float4* out);
+void launch_coarse_dump(const Launch& L, int m, hipStream_t s, const uint32_t* cstart,
+                        const float4* vel, const float4* thq, float* out, int sx, int sy, int sz,
+                        int cx, int cy, int cz);
+void launch_column_diag(const Launch& L, int m, int ncol, hipStream_t s, const uint32_t* cstart,
+                        const float4* thq, const float4* vel, float4* out);
 
 int radix_tile();
-// Sorts the Morton keys of pos over bits [0, P.end_bit). keys0 receives the unsorted keys;
+// Stable LSD radix sort of the keys in keys0 (bits [0, end_bit)) with their indices.
 // (kA,vA) and (kB,vB) are ping-pong buffers; bh, bo hold 256*nblk counts, dtot 256 totals.
 // On return *keys_sorted and *perm point into kA/vA or kB/vB.
-void launch_radix_sort(const DevParams& P, hipStream_t s, const float4* pos, uint32_t* keys0,
-                       uint32_t* kA, uint32_t* vA, uint32_t* kB, uint32_t* vB, uint32_t* bh,
-                       uint32_t* bo, uint32_t* dtot, uint32_t** keys_sorted, uint32_t** perm);
+void launch_radix_sort(hipStream_t s, int N, int end_bit, const uint32_t* keys0, uint32_t* kA,
+                       uint32_t* vA, uint32_t* kB, uint32_t* vB, uint32_t* bh, uint32_t* bo,
+                       uint32_t* dtot, uint32_t** keys_sorted, uint32_t** perm);
 
 }  // namespace asph

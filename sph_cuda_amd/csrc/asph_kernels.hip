@@ -1,14 +1,15 @@
-// asph_kernels.hip - gfx950 kernels of one KDK step of the anisotropic-SPH model.
+// asph_kernels.hip - the streaming kernels of one KDK step (kick/drift, rain sedimentation),
+// the sort helpers (Morton keys, gather, cell start table) and the diagnostics.
 //
-// Launch shape: one work-item per particle (or cell), 256 work-items (4 wave64) per
-// workgroup. Particles are stored in Morton order after every sort, so the 64 particles of a
-// wavefront occupy a 4x4x4 block of cells and their neighbour stencils overlap: the
-// neighbour reads of a wavefront hit the same cache lines.
+// Launch shape: 256 work-items (4 wave64) per workgroup. Every workgroup reads its member's
+// parameters from the member table through the block map (one member per block), so one launch
+// serves a single domain or a whole ensemble. Particles are stored in Morton order after every
+// sort, so the 64 particles of a wavefront occupy a 4x4x4 block of cells.
 //
-// Neighbour structure: cstart[k] = index of the first particle whose Morton key is >= k, for
-// every key k in [0, 2^end_bit], so cell k holds particles [cstart[k], cstart[k+1]) -- one
-// 4-byte start table in key order instead of the reference's int2 cellRange in row-major
-// order (asph_h100.cu:257-278), written with coalesced stores and no clearing pass.
+// Neighbour structure: cstart[k] = index of the first particle whose key is >= k, for every key
+// k in [0, nkeys], so cell k holds particles [cstart[k], cstart[k+1]) -- one 4-byte start table
+// in key order instead of the reference's int2 cellRange in row-major order
+// (asph_h100.cu:257-278), written with coalesced stores and no clearing pass.
 //
 // The two neighbour-loop kernels live in asph_nbr.hip.
 #include "asph_kernels.h"
@@ -21,40 +22,45 @@ constexpr int TPB = 256;
 
 // ---------------------------------------------------------------------------------------------
 // kick(dt/2) + drift, asph_h100.cu:535-561. Also clears the per-step rain deposit grid
-// (consumed by k_rain of the previous step).
+// (consumed by k_rain of the previous step) and records the step's drift bound.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(TPB) void k_kick_drift(DevParams P, float4* __restrict__ pos,
-                                                    float4* __restrict__ vel,
+__global__ __launch_bounds__(TPB) void k_kick_drift(const MemberDesc* __restrict__ mem,
+                                                    const BlockMap* __restrict__ pmap,
+                                                    float4* __restrict__ pos, float4* __restrict__ vel,
                                                     const float4* __restrict__ acc,
                                                     int64_t* __restrict__ dep,
                                                     uint32_t* __restrict__ disp_slots, float hdt,
                                                     float dt) {
-    int i = blockIdx.x * TPB + threadIdx.x;
+    const BlockMap bm = pmap[blockIdx.x];
+    const MemberDesc& M = mem[bm.m];
+    const DevParams& P = M.P;
+    const int il = bm.base + threadIdx.x;
     float mx = 0.f, my = 0.f, mz = 0.f;
-    if (i < P.N) {
-    float4 v = vel[i];
-    float4 a = acc[i];
-    v.x = __builtin_fmaf(a.x, hdt, v.x);
-    v.y = __builtin_fmaf(a.y, hdt, v.y);
-    v.z = __builtin_fmaf(a.z, hdt, v.z);
-    float4 p = pos[i];
-    p.x = __builtin_fmaf(v.x, dt, p.x);
-    p.y = __builtin_fmaf(v.y, dt, p.y);
-    p.z = __builtin_fmaf(v.z, dt, p.z);
-    // periodic wrap (the reference's while loops, bounded so a non-finite position cannot spin)
-    for (int k = 0; k < 8 && p.x < 0.f; ++k) p.x += P.Lx;
-    for (int k = 0; k < 8 && p.x >= P.Lx; ++k) p.x -= P.Lx;
-    for (int k = 0; k < 8 && p.y < 0.f; ++k) p.y += P.Ly;
-    for (int k = 0; k < 8 && p.y >= P.Ly; ++k) p.y -= P.Ly;
-    if (p.z < 0.f) { p.z = -p.z; v.z = fabsf(v.z); }
-    if (p.z > P.Lz) { p.z = 2.f * P.Lz - p.z; v.z = -fabsf(v.z); }
-    pos[i] = p;
-    vel[i] = v;
-    if (i < P.ncell) dep[i] = 0;
-    // drift bound of this step in cells per axis (|dx| <= |v dt|, also across a wrap or a wall)
-    mx = fabsf(v.x * dt) * P.inv_dx;
-    my = fabsf(v.y * dt) * P.inv_dy;
-    mz = fabsf(v.z * dt) * P.inv_dz;
+    if (il < P.N) {
+        const int i = M.p0 + il;
+        float4 v = vel[i];
+        float4 a = acc[i];
+        v.x = __builtin_fmaf(a.x, hdt, v.x);
+        v.y = __builtin_fmaf(a.y, hdt, v.y);
+        v.z = __builtin_fmaf(a.z, hdt, v.z);
+        float4 p = pos[i];
+        p.x = __builtin_fmaf(v.x, dt, p.x);
+        p.y = __builtin_fmaf(v.y, dt, p.y);
+        p.z = __builtin_fmaf(v.z, dt, p.z);
+        // periodic wrap (the reference's while loops, bounded so a non-finite position cannot spin)
+        for (int k = 0; k < 8 && p.x < 0.f; ++k) p.x += P.Lx;
+        for (int k = 0; k < 8 && p.x >= P.Lx; ++k) p.x -= P.Lx;
+        for (int k = 0; k < 8 && p.y < 0.f; ++k) p.y += P.Ly;
+        for (int k = 0; k < 8 && p.y >= P.Ly; ++k) p.y -= P.Ly;
+        if (p.z < 0.f) { p.z = -p.z; v.z = fabsf(v.z); }
+        if (p.z > P.Lz) { p.z = 2.f * P.Lz - p.z; v.z = -fabsf(v.z); }
+        pos[i] = p;
+        vel[i] = v;
+        dep[i] = 0;   // cells = particles per member: cell index = particle index
+        // drift bound of this step in cells per axis (|dx| <= |v dt|, also across a wrap or a wall)
+        mx = fabsf(v.x * dt) * P.inv_dx;
+        my = fabsf(v.y * dt) * P.inv_dy;
+        mz = fabsf(v.z * dt) * P.inv_dz;
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -70,21 +76,26 @@ __global__ __launch_bounds__(TPB) void k_kick_drift(DevParams P, float4* __restr
     }
 }
 
-// Drift bookkeeping: disp[a] accumulates the per-step maxima (slots) since the last sort; the
-// neighbour kernels read disp + max(slots) as the bound on how far a particle has left the
-// cell it is listed in. reset != 0 clears both (after a sort).
-__global__ void k_disp_fold(float* __restrict__ disp, uint32_t* __restrict__ slots, int reset) {
-    int t = threadIdx.x;   // 64 threads
+// Per-step bookkeeping, one wavefront: folds the drift maxima of the last step into the bound
+// since the sort (reset != 0 after a sort clears it) and sets or advances the simulated-time
+// clock read by the kernels (mode 0: clock = t, mode 1: clock += dt, mode 2: leave).
+__global__ void k_step_begin(float* __restrict__ disp, uint32_t* __restrict__ slots, int reset,
+                             double* __restrict__ clock, int mode, double t, float dt) {
+    int tid = threadIdx.x;   // 64 threads
     float m[3];
 #pragma unroll
     for (int a = 0; a < 3; ++a) {
-        float v = __uint_as_float(slots[a * 64 + t]);
+        float v = __uint_as_float(slots[a * 64 + tid]);
 #pragma unroll
         for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
         m[a] = v;
-        slots[a * 64 + t] = 0u;
+        slots[a * 64 + tid] = 0u;
     }
-    if (t < 3) disp[t] = reset ? 0.f : disp[t] + m[t];
+    if (tid < 3) disp[tid] = reset ? 0.f : disp[tid] + m[tid];
+    if (tid == 0) {
+        if (mode == 0) clock[0] = t;
+        else if (mode == 1) clock[0] = clock[0] + (double)dt;
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -94,32 +105,38 @@ __global__ void k_disp_fold(float* __restrict__ disp, uint32_t* __restrict__ slo
 // a negative cell (rain evaporation exceeding the cell's rain) is clipped to zero and the
 // clipped amount is counted.
 // ---------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(TPB) void k_rain(DevParams P, const int64_t* __restrict__ rain_in,
+__global__ __launch_bounds__(TPB) void k_rain(const MemberDesc* __restrict__ mem,
+                                              const BlockMap* __restrict__ pmap,
+                                              const int64_t* __restrict__ rain_in,
                                               const int64_t* __restrict__ dep,
                                               int64_t* __restrict__ rain_out,
                                               int64_t* __restrict__ precip,
-                                              unsigned long long* __restrict__ clipped, float frac,
-                                              int sediment) {
-    int c = blockIdx.x * TPB + threadIdx.x;
-    if (c >= P.ncell) return;
+                                              unsigned long long* __restrict__ clipped, float dt) {
+    const BlockMap bm = pmap[blockIdx.x];
+    const MemberDesc& M = mem[bm.m];
+    const DevParams& P = M.P;
+    const int cl = bm.base + threadIdx.x;
+    if (cl >= P.ncell) return;
+    const int c = M.p0 + cl;
     int64_t cur = rain_in[c] + dep[c];
     if (cur < 0) {
         atomicAdd(clipped, (unsigned long long)(-cur));
         cur = 0;
     }
-    if (!sediment) {
+    if (!(P.flags & ASPH_F_SEDIMENT)) {
         rain_out[c] = cur;
         return;
     }
+    const float frac = rain_frac(P, dt);
     int64_t om = (int64_t)((double)cur * (double)frac);
     int64_t im = 0;
-    if (c + P.ncol < P.ncell) {
+    if (cl + P.ncol < P.ncell) {
         int64_t ca = rain_in[c + P.ncol] + dep[c + P.ncol];
         if (ca < 0) ca = 0;
         im = (int64_t)((double)ca * (double)frac);
     }
     rain_out[c] = cur - om + im;
-    if (c < P.ncol) precip[c] += om;
+    if (cl < P.ncol) precip[M.col0 + cl] += om;
 }
 
 // Merge deposits into the rain grid without advection (init-time force evaluation, :918).
@@ -129,6 +146,24 @@ __global__ __launch_bounds__(TPB) void k_rain_merge(int n, int64_t* __restrict__
     if (c >= n) return;
     rain[c] += dep[c];
     dep[c] = 0;
+}
+
+// Morton keys of the current positions (kernel_morton_keys, asph_h100.cu:229-240), prefixed
+// with the member's key offset.
+__global__ __launch_bounds__(TPB) void k_keys(const MemberDesc* __restrict__ mem,
+                                              const BlockMap* __restrict__ pmap,
+                                              const float4* __restrict__ pos,
+                                              uint32_t* __restrict__ keys) {
+    const BlockMap bm = pmap[blockIdx.x];
+    const MemberDesc& M = mem[bm.m];
+    const DevParams& P = M.P;
+    const int il = bm.base + threadIdx.x;
+    if (il >= P.N) return;
+    const int i = M.p0 + il;
+    float4 p = pos[i];
+    int ix, iy, iz;
+    cell_trunc(P, p.x, p.y, p.z, ix, iy, iz);
+    keys[i] = M.k0 | morton3((uint32_t)ix, (uint32_t)iy, (uint32_t)iz);
 }
 
 // Gather, asph_h100.cu:245-256 (+ acc when ASPH_F_GATHER_ACC).
@@ -165,10 +200,17 @@ __global__ __launch_bounds__(TPB) void k_cell_start(int N, uint32_t nkeys,
         cstart[q] = (uint32_t)N;
 }
 
-// kernel_clamp_vel, asph_h100.cu:669-675.
-__global__ __launch_bounds__(TPB) void k_clamp_vel(int N, float4* __restrict__ vel, float vmax) {
-    int i = blockIdx.x * TPB + threadIdx.x;
-    if (i >= N) return;
+// kernel_clamp_vel, asph_h100.cu:669-675: |v| <= vmax_abs, or <= cs_frac * cs of each member.
+__global__ __launch_bounds__(TPB) void k_clamp_vel(const MemberDesc* __restrict__ mem,
+                                                   const BlockMap* __restrict__ pmap,
+                                                   float4* __restrict__ vel, float vmax_abs,
+                                                   float cs_frac) {
+    const BlockMap bm = pmap[blockIdx.x];
+    const MemberDesc& M = mem[bm.m];
+    const int il = bm.base + threadIdx.x;
+    if (il >= M.P.N) return;
+    const int i = M.p0 + il;
+    const float vmax = vmax_abs > 0.f ? vmax_abs : cs_frac * M.P.cs;
     float4 v = vel[i];
     float s2 = v.x * v.x + v.y * v.y + v.z * v.z;
     if (s2 > vmax * vmax) {
@@ -178,46 +220,58 @@ __global__ __launch_bounds__(TPB) void k_clamp_vel(int N, float4* __restrict__ v
     }
 }
 
-// kernel_audit, asph_h100.cu:650-668 (FP64 accumulation) + rain water.
-__global__ __launch_bounds__(TPB) void k_audit(DevParams P, const float4* __restrict__ pos,
+// kernel_audit, asph_h100.cu:650-668 (FP64 accumulation, per member scaled by its particle
+// mass) + rain water. tot[0..6] as asph_audit.
+__global__ __launch_bounds__(TPB) void k_audit(const MemberDesc* __restrict__ mem,
+                                               const BlockMap* __restrict__ pmap,
+                                               const float4* __restrict__ pos,
                                                const float4* __restrict__ vel,
                                                const float4* __restrict__ thq,
                                                const int64_t* __restrict__ rain,
                                                const int64_t* __restrict__ dep,
                                                const int64_t* __restrict__ precip,
                                                double* __restrict__ tot) {
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    int stride = gridDim.x * TPB;
-    for (int i = blockIdx.x * TPB + threadIdx.x; i < P.N; i += stride) {
+    const BlockMap bm = pmap[blockIdx.x];
+    const MemberDesc& M = mem[bm.m];
+    const DevParams& P = M.P;
+    const int il = bm.base + threadIdx.x;
+    double s[7] = {0, 0, 0, 0, 0, 0, 0};
+    if (il < P.N) {
+        const int i = M.p0 + il;
         float4 p = pos[i], v = vel[i], t = thq[i];
         float Pi = asphm::fpow(fmaxf(pres_ref(P, p.z), 1.f) / P0, P.rd_cp);
-        s[0] += 0.5 * (double)(v.x * v.x + v.y * v.y + v.z * v.z);
-        s[1] += (double)(GRAV * p.z);
-        s[2] += (double)((CP * t.x) * Pi);
-        s[3] += (double)(t.y + t.z);
-        s[4] += (double)v.z;
+        const double m = (double)P.pmass;
+        s[0] = m * (0.5 * (double)(v.x * v.x + v.y * v.y + v.z * v.z));
+        s[1] = m * (double)(GRAV * p.z);
+        s[2] = m * (double)((CP * t.x) * Pi);
+        s[3] = m * (double)(t.y + t.z);
+        s[4] = m * (double)v.z;
+        s[5] = m;
+        double r = (double)(rain[i] + (dep ? dep[i] : 0));
+        if (il < P.ncol) r += (double)precip[M.col0 + il];
+        s[6] = m * r * FX_INV;
     }
-    for (int c = blockIdx.x * TPB + threadIdx.x; c < P.ncell; c += stride)
-        s[5] += (double)(rain[c] + (dep ? dep[c] : 0));
-    for (int c = blockIdx.x * TPB + threadIdx.x; c < P.ncol; c += stride) s[5] += (double)precip[c];
-    __shared__ double red[6][TPB];
+    __shared__ double red[7][TPB];
 #pragma unroll
-    for (int k = 0; k < 6; ++k) red[k][threadIdx.x] = s[k];
+    for (int k = 0; k < 7; ++k) red[k][threadIdx.x] = s[k];
     __syncthreads();
     for (int w = TPB / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w)
-            for (int k = 0; k < 6; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
+            for (int k = 0; k < 7; ++k) red[k][threadIdx.x] += red[k][threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x < 6) atomicAdd(&tot[threadIdx.x], red[threadIdx.x][0]);
+    if (threadIdx.x < 7) atomicAdd(&tot[threadIdx.x], red[threadIdx.x][0]);
 }
 
-// kernel_coarse_dump, asph_h100.cu:680-703 (cells visited in the reference's order).
-__global__ __launch_bounds__(TPB) void k_coarse_dump(DevParams P, const uint32_t* __restrict__ cstart,
+// kernel_coarse_dump, asph_h100.cu:680-703, for member m (cells in the reference's order).
+__global__ __launch_bounds__(TPB) void k_coarse_dump(const MemberDesc* __restrict__ mem, int m,
+                                                     const uint32_t* __restrict__ cstart,
                                                      const float4* __restrict__ vel,
                                                      const float4* __restrict__ thq,
                                                      float* __restrict__ out, int sx, int sy, int sz,
                                                      int cx, int cy, int cz) {
+    const MemberDesc& M = mem[m];
+    const DevParams& P = M.P;
     int idx = blockIdx.x * TPB + threadIdx.x;
     if (idx >= cx * cy * cz) return;
     int cxx = idx % cx, cyy = (idx / cx) % cy, czz = idx / (cx * cy);
@@ -227,7 +281,7 @@ __global__ __launch_bounds__(TPB) void k_coarse_dump(DevParams P, const uint32_t
     for (int iz = czz * sz; iz < iz1; ++iz)
         for (int iy = cyy * sy; iy < iy1; ++iy)
             for (int ix = cxx * sx; ix < ix1; ++ix) {
-                uint32_t k = morton3((uint32_t)ix, (uint32_t)iy, (uint32_t)iz);
+                uint32_t k = M.k0 | morton3((uint32_t)ix, (uint32_t)iy, (uint32_t)iz);
                 for (uint32_t j = cstart[k]; j < cstart[k + 1]; ++j) {
                     float4 t = thq[j], v = vel[j];
                     a[0] += t.x; a[1] += t.y; a[2] += t.z;
@@ -239,18 +293,21 @@ __global__ __launch_bounds__(TPB) void k_coarse_dump(DevParams P, const uint32_t
     for (int c = 0; c < 6; ++c) out[(size_t)idx * 6 + c] = a[c] * inv;
 }
 
-// kernel_column_diag, asph_h100.cu:704-723.
-__global__ __launch_bounds__(TPB) void k_column_diag(DevParams P, const uint32_t* __restrict__ cstart,
+// kernel_column_diag, asph_h100.cu:704-723, for member m.
+__global__ __launch_bounds__(TPB) void k_column_diag(const MemberDesc* __restrict__ mem, int m,
+                                                     const uint32_t* __restrict__ cstart,
                                                      const float4* __restrict__ thq,
                                                      const float4* __restrict__ vel,
                                                      float4* __restrict__ out) {
+    const MemberDesc& M = mem[m];
+    const DevParams& P = M.P;
     int col = blockIdx.x * TPB + threadIdx.x;
     if (col >= P.ncol) return;
     int iy = col / P.nx, ix = col % P.nx;
     float th = 0.f, qv = 0.f, qlmax = 0.f, wmax = 0.f;
     int n = 0;
     for (int k = 0; k < P.nz; ++k) {
-        uint32_t key = morton3((uint32_t)ix, (uint32_t)iy, (uint32_t)k);
+        uint32_t key = M.k0 | morton3((uint32_t)ix, (uint32_t)iy, (uint32_t)k);
         for (uint32_t j = cstart[key]; j < cstart[key + 1]; ++j) {
             float4 t = thq[j];
             th += t.x; qv += t.y;
@@ -267,27 +324,32 @@ inline int nblocks(int64_t n) { return (int)((n + TPB - 1) / TPB); }
 
 }  // namespace
 
+int particle_block_size() { return TPB; }
+
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
-void launch_kick_drift(const DevParams& P, hipStream_t s, float4* pos, float4* vel,
-                       const float4* acc, int64_t* dep, float* disp, uint32_t* disp_slots, float dt) {
-    k_disp_fold<<<1, 64, 0, s>>>(disp, disp_slots, 0);
-    k_kick_drift<<<nblocks(P.N), TPB, 0, s>>>(P, pos, vel, acc, dep, disp_slots, 0.5f * dt, dt);
+void launch_step_begin(hipStream_t s, float* disp, uint32_t* disp_slots, int reset, double* clock,
+                       int clock_mode, double t, float dt) {
+    k_step_begin<<<1, 64, 0, s>>>(disp, disp_slots, reset, clock, clock_mode, t, dt);
 }
 
-void launch_disp_reset(hipStream_t s, float* disp, uint32_t* disp_slots) {
-    k_disp_fold<<<1, 64, 0, s>>>(disp, disp_slots, 1);
+void launch_kick_drift(const Launch& L, hipStream_t s, float4* pos, float4* vel, const float4* acc,
+                       int64_t* dep, uint32_t* disp_slots, float dt) {
+    k_kick_drift<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, pos, vel, acc, dep, disp_slots, 0.5f * dt, dt);
 }
 
-void launch_rain(const DevParams& P, hipStream_t s, const int64_t* rain_in, const int64_t* dep,
-                 int64_t* rain_out, int64_t* precip, unsigned long long* clipped, float frac,
-                 int sediment) {
-    k_rain<<<nblocks(P.ncell), TPB, 0, s>>>(P, rain_in, dep, rain_out, precip, clipped, frac, sediment);
+void launch_rain(const Launch& L, hipStream_t s, const int64_t* rain_in, const int64_t* dep,
+                 int64_t* rain_out, int64_t* precip, unsigned long long* clipped, float dt) {
+    k_rain<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, rain_in, dep, rain_out, precip, clipped, dt);
 }
 
 void launch_rain_merge(hipStream_t s, int n, int64_t* rain, int64_t* dep) {
     k_rain_merge<<<nblocks(n), TPB, 0, s>>>(n, rain, dep);
+}
+
+void launch_keys(const Launch& L, hipStream_t s, const float4* pos, uint32_t* keys) {
+    k_keys<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, pos, keys);
 }
 
 void launch_gather(hipStream_t s, int N, const uint32_t* perm, const float4* pA, float4* pB,
@@ -300,24 +362,26 @@ void launch_cell_start(hipStream_t s, int N, uint32_t nkeys, const uint32_t* key
     k_cell_start<<<nblocks(N), TPB, 0, s>>>(N, nkeys, keys, cstart);
 }
 
-void launch_clamp_vel(hipStream_t s, int N, float4* vel, float vmax) {
-    k_clamp_vel<<<nblocks(N), TPB, 0, s>>>(N, vel, vmax);
+void launch_clamp_vel(const Launch& L, hipStream_t s, float4* vel, float vmax, float cs_frac) {
+    k_clamp_vel<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, vel, vmax, cs_frac);
 }
 
-void launch_audit(const DevParams& P, hipStream_t s, const float4* pos, const float4* vel,
+void launch_audit(const Launch& L, hipStream_t s, const float4* pos, const float4* vel,
                   const float4* thq, const int64_t* rain, const int64_t* dep, const int64_t* precip,
                   double* tot) {
-    k_audit<<<1024, TPB, 0, s>>>(P, pos, vel, thq, rain, dep, precip, tot);
+    k_audit<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, pos, vel, thq, rain, dep, precip, tot);
 }
 
-void launch_coarse_dump(const DevParams& P, hipStream_t s, const uint32_t* cstart, const float4* vel,
-                        const float4* thq, float* out, int sx, int sy, int sz, int cx, int cy, int cz) {
-    k_coarse_dump<<<nblocks((int64_t)cx * cy * cz), TPB, 0, s>>>(P, cstart, vel, thq, out, sx, sy, sz, cx, cy, cz);
+void launch_coarse_dump(const Launch& L, int m, hipStream_t s, const uint32_t* cstart,
+                        const float4* vel, const float4* thq, float* out, int sx, int sy, int sz,
+                        int cx, int cy, int cz) {
+    k_coarse_dump<<<nblocks((int64_t)cx * cy * cz), TPB, 0, s>>>(L.mem, m, cstart, vel, thq, out, sx,
+                                                                 sy, sz, cx, cy, cz);
 }
 
-void launch_column_diag(const DevParams& P, hipStream_t s, const uint32_t* cstart, const float4* thq,
-                        const float4* vel, float4* out) {
-    k_column_diag<<<nblocks(P.ncol), TPB, 0, s>>>(P, cstart, thq, vel, out);
+void launch_column_diag(const Launch& L, int m, int ncol, hipStream_t s, const uint32_t* cstart,
+                        const float4* thq, const float4* vel, float4* out) {
+    k_column_diag<<<nblocks(ncol), TPB, 0, s>>>(L.mem, m, cstart, thq, vel, out);
 }
 
 }  // namespace asph

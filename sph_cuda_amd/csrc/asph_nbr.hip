@@ -2,8 +2,9 @@
 // fused with the EOS and warm-rain microphysics, and momentum fused with the theta update, the
 // second half kick, the sponge and the max|v| reduction.
 //
-// Work decomposition (mode A): one 512-thread workgroup per 8x8x8-cell brick of the Morton key
-// space (keys [b*512, b*512+512)). The workgroup owns the particles whose CURRENT cell lies in
+// Work decomposition (mode A): one 512-thread workgroup per 8x8x8-cell brick of a member's
+// Morton key space (local keys [base, base+512), listed in the brick map; an ensemble launches
+// the bricks of all its members together). The workgroup owns the particles whose CURRENT cell lies in
 // its brick and stages, into LDS, every particle LISTED (by the last sort) in the brick's halo
 // (brick +- stencil cells), stored in row-major order of the halo cells with a per-cell start
 // table. For an owned particle i each stencil row (jz, jy) is then one contiguous LDS span, and
@@ -65,7 +66,7 @@ template <int S> __device__ __forceinline__ void zrange(const DevParams& P, int 
 // Thread-per-particle loops over the global cell table (mode B and overflow fallback).
 // ---------------------------------------------------------------------------------------------
 template <int S>
-__device__ void dens_sum_global(const DevParams& P, float4 pi, const float4* __restrict__ pos,
+__device__ void dens_sum_global(const DevParams& P, uint32_t k0, float4 pi, const float4* __restrict__ pos,
                                 const float* __restrict__ rho_old, const uint32_t* __restrict__ cstart,
                                 float& S1, float& S2) {
     int ix, iy, iz, z0, z1;
@@ -74,7 +75,7 @@ __device__ void dens_sum_global(const DevParams& P, float4 pi, const float4* __r
     S1 = 0.f;
     S2 = 0.f;
     for (int jz = z0; jz <= z1; ++jz) {
-        uint32_t kz = part1by2((uint32_t)jz) << 2;
+        uint32_t kz = k0 | (part1by2((uint32_t)jz) << 2);
         for (int jy = iy - S; jy <= iy + S; ++jy) {
             uint32_t kyz = kz | (part1by2((uint32_t)wrapi(jy, P.ny)) << 1);
             for (int jx = ix - S; jx <= ix + S; ++jx) {
@@ -131,7 +132,7 @@ __device__ __forceinline__ void mom_pair(const DevParams& P, float rx, float ry,
 }
 
 template <int S>
-__device__ void mom_sum_global(const DevParams& P, float4 pi, float4 vi, float rhoi, float pri,
+__device__ void mom_sum_global(const DevParams& P, uint32_t k0, float4 pi, float4 vi, float rhoi, float pri,
                                const float4* __restrict__ pos, const float4* __restrict__ vel,
                                const float* __restrict__ rho, const float* __restrict__ prs,
                                const uint32_t* __restrict__ cstart, MomAcc& A) {
@@ -140,7 +141,7 @@ __device__ void mom_sum_global(const DevParams& P, float4 pi, float4 vi, float r
     zrange<S>(P, iz, z0, z1);
     const float inv_rhoi = 1.0f / rhoi;
     for (int jz = z0; jz <= z1; ++jz) {
-        uint32_t kz = part1by2((uint32_t)jz) << 2;
+        uint32_t kz = k0 | (part1by2((uint32_t)jz) << 2);
         for (int jy = iy - S; jy <= iy + S; ++jy) {
             uint32_t kyz = kz | (part1by2((uint32_t)wrapi(jy, P.ny)) << 1);
             for (int jx = ix - S; jx <= ix + S; ++jx) {
@@ -168,8 +169,13 @@ __device__ void mom_sum_global(const DevParams& P, float4 pi, float4 vi, float r
 // density ratio + clamp (:331-332), surface flux (:606-626), EOS + microphysics (:353-401),
 // Kessler accretion and rain evaporation. Writes rho_new, thq, prs and p' into vel.w (the
 // momentum kernel reads (v, p') of a neighbour with one load; the reference leaves vel.w unused).
-__device__ void dens_epilogue(const DevParams& P, const DensArgs& A, int i, float4 pi, float S1,
-                              float S2) {
+struct DensConst {
+    float auto_f, Ts, qs_sfc;
+    int p0;          // member's first particle (= first cell)
+};
+
+__device__ void dens_epilogue(const DevParams& P, const DensArgs& A, const DensConst& K, int i,
+                              float4 pi, float S1, float S2) {
     const float z = pi.z;
     const float ex = exner_ref(P, z);
     const float pref = pres_of_exner(P, ex);
@@ -180,11 +186,12 @@ __device__ void dens_epilogue(const DevParams& P, const DensArgs& A, int i, floa
     if (!(A.bits & 1)) return;
 
     float4 t = A.thq[i];
-    if ((A.bits & 2) && (A.keys_sorted[i] & 0x24924924u) == 0u && z < P.dz) {
+    if ((A.bits & 2) && (P.flags & ASPH_F_SURFACE_FLUX) && (A.keys_sorted[i] & 0x24924924u) == 0u &&
+        z < P.dz) {
         float Pi = asphm::fpow(fmaxf(pref, 1.0f) / P0, P.rd_cp);
         float Ta = t.x * Pi;
-        t.x = t.x + (((P.c_h * P.v_bg) * (A.Ts - Ta)) * A.dt) / P.dz;
-        t.y = t.y + ((((P.beta_moist * P.c_e) * P.v_bg) * (A.qs_sfc - t.y)) * A.dt) / P.dz;
+        t.x = t.x + (((P.c_h * P.v_bg) * (K.Ts - Ta)) * A.dt) / P.dz;
+        t.y = t.y + ((((P.beta_moist * P.c_e) * P.v_bg) * (K.qs_sfc - t.y)) * A.dt) / P.dz;
         t.x = fminf(fmaxf(t.x, 100.f), P.theta_cap);
         t.y = fminf(fmaxf(t.y, 0.f), P.qv_cap);
     }
@@ -217,13 +224,13 @@ __device__ void dens_epilogue(const DevParams& P, const DensArgs& A, int i, floa
     }
     int64_t dfx = 0;
     if ((P.flags & ASPH_F_AUTOCONV) && qc > P.qc_auto) {
-        float dqr = (qc - P.qc_auto) * A.auto_f;
+        float dqr = (qc - P.qc_auto) * K.auto_f;
         qc -= dqr;
         dfx += to_fx(dqr);
     }
     int cid = -1;
     if (P.flags & (ASPH_F_ACCRETION | ASPH_F_RAIN_EVAP)) {
-        cid = cell_of(P, pi.x, pi.y, pi.z);
+        cid = K.p0 + cell_of(P, pi.x, pi.y, pi.z);
         float rf = rain_fx_to_float(A.rain[cid]);
         float qr = (rf * P.mm) / r;
         if ((P.flags & ASPH_F_ACCRETION) && qc > 0.f && qr > 0.f) {
@@ -247,7 +254,7 @@ __device__ void dens_epilogue(const DevParams& P, const DensArgs& A, int i, floa
         }
     }
     if (dfx != 0) {
-        if (cid < 0) cid = cell_of(P, pi.x, pi.y, pi.z);
+        if (cid < 0) cid = K.p0 + cell_of(P, pi.x, pi.y, pi.z);
         atomicAdd((unsigned long long*)&A.dep[cid], (unsigned long long)dfx);
     }
     th = fminf(fmaxf(th, 100.0f), P.theta_cap);
@@ -318,9 +325,8 @@ struct BrickGeo {
     bool wraps;  // the halo crosses a periodic face
 };
 
-__device__ __forceinline__ BrickGeo brick_geo(const DevParams& P, int S) {
+__device__ __forceinline__ BrickGeo brick_geo(const DevParams& P, int S, uint32_t kb) {
     BrickGeo g;
-    uint32_t kb = (uint32_t)blockIdx.x << NB;
     g.bx0 = (int)compact1by2(kb);
     g.by0 = (int)compact1by2(kb >> 1);
     g.bz0 = (int)compact1by2(kb >> 2);
@@ -360,7 +366,7 @@ __device__ void prologue(const DevParams& P, const float* disp, const uint32_t* 
 
 // Stages the halo of this workgroup's brick; returns false if it does not fit.
 template <int S, bool MOM, class Rec>
-__device__ bool stage(const DevParams& P, const BrickGeo& g, const float4* __restrict__ pos,
+__device__ bool stage(const DevParams& P, uint32_t k0, const BrickGeo& g, const float4* __restrict__ pos,
                       const uint32_t* __restrict__ cstart, Smem<S, MOM>& sm, Rec rec) {
     constexpr int H = Halo<S>::H;
     constexpr int HC = Halo<S>::HC;
@@ -380,8 +386,8 @@ __device__ bool stage(const DevParams& P, const BrickGeo& g, const float4* __res
             int hx = c % H, hy = (c / H) % H, hz = c / (H * H);
             int gz = hz0 + hz;
             if (gz >= 0 && gz < P.nz) {
-                uint32_t key = morton3((uint32_t)wrapi(hx0 + hx, P.nx), (uint32_t)wrapi(hy0 + hy, P.ny),
-                                       (uint32_t)gz);
+                uint32_t key = k0 | morton3((uint32_t)wrapi(hx0 + hx, P.nx), (uint32_t)wrapi(hy0 + hy, P.ny),
+                                            (uint32_t)gz);
                 uint32_t s = cstart[key];
                 start[k] = s;
                 cnt[k] = cstart[key + 1] - s;
@@ -444,13 +450,13 @@ __device__ bool stage(const DevParams& P, const BrickGeo& g, const float4* __res
 // Calls fn(j) for every particle whose current cell is in the brick, found among the particles
 // listed within one cell of it (overflow fallback; thread-strided).
 template <int S, class Fn>
-__device__ void for_each_owned_global(const DevParams& P, const BrickGeo& g, const float4* __restrict__ pos,
+__device__ void for_each_owned_global(const DevParams& P, uint32_t k0, const BrickGeo& g, const float4* __restrict__ pos,
                                       const uint32_t* __restrict__ cstart, Fn fn) {
     constexpr int W = BS + 2;
     for (int c = threadIdx.x; c < W * W * W; c += BT) {
         int ux = g.bx0 - 1 + c % W, uy = g.by0 - 1 + (c / W) % W, gz = g.bz0 - 1 + c / (W * W);
         if (gz < 0 || gz >= P.nz) continue;
-        uint32_t key = morton3((uint32_t)wrapi(ux, P.nx), (uint32_t)wrapi(uy, P.ny), (uint32_t)gz);
+        uint32_t key = k0 | morton3((uint32_t)wrapi(ux, P.nx), (uint32_t)wrapi(uy, P.ny), (uint32_t)gz);
         for (uint32_t jg = cstart[key]; jg < cstart[key + 1]; ++jg) {
             float4 pj = pos[jg];
             int ix, iy, iz;
@@ -497,8 +503,8 @@ __device__ __forceinline__ void walk_rows(const DevParams& P, const BrickGeo& g,
 // Density + EOS + microphysics.
 // ---------------------------------------------------------------------------------------------
 template <int S, bool WRAP>
-__device__ void dens_own(const DevParams& P, const DensArgs& A, const BrickGeo& g, Smem<S, false>& sm,
-                         int i, bool active) {
+__device__ void dens_own(const DevParams& P, const DensArgs& A, const DensConst& K, const BrickGeo& g,
+                         Smem<S, false>& sm, int i, bool active) {
     const int tid = threadIdx.x;
     float4 pi = active ? A.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
     float S1 = 0.f, S2 = 0.f;
@@ -536,35 +542,45 @@ __device__ void dens_own(const DevParams& P, const DensArgs& A, const BrickGeo& 
         });
     }
     flush();
-    if (active) dens_epilogue(P, A, i, pi, S1, S2);
+    if (active) dens_epilogue(P, A, K, i, pi, S1, S2);
 }
 
 template <int S>
-__global__ __launch_bounds__(BT) void k_density_eos_brick(DevParams P, DensArgs A) {
+__global__ __launch_bounds__(BT) void k_density_eos_brick(const MemberDesc* __restrict__ mem,
+                                                          const BlockMap* __restrict__ bmap, DensArgs A) {
     __shared__ Smem<S, false> sm;
     const int tid = threadIdx.x;
-    if (blockIdx.x == 0 && tid < 3 * 64) A.maxv[tid] = 0u;
+    const BlockMap bm = bmap[blockIdx.x];
+    const MemberDesc& MD = mem[bm.m];
+    const DevParams P = MD.P;
+    const uint32_t k0 = MD.k0;
+    if (bm.ord == 0 && tid < 3 * 64) A.maxv[bm.m * 192 + tid] = 0u;   // this member's max|v| slots
+    DensConst K;
+    K.p0 = MD.p0;
+    K.auto_f = auto_factor(P, A.dt);
+    sfc_params(P, A.clock ? A.clock[0] : A.simt, K.Ts, K.qs_sfc);
     prologue<S, false>(P, A.disp, A.disp_slots, sm);
-    if (!sm.modeA) {  // mode B: particle per thread by index
-        int i = blockIdx.x * BT + tid;
-        if (i < P.N) {
+    if (!sm.modeA) {  // mode B: particle per thread by index within the member
+        int il = bm.ord * BT + tid;
+        if (il < P.N) {
+            int i = MD.p0 + il;
             float4 pi = A.pos[i];
             float S1, S2;
-            dens_sum_global<S>(P, pi, A.pos, A.rho_old, A.cstart, S1, S2);
-            dens_epilogue(P, A, i, pi, S1, S2);
+            dens_sum_global<S>(P, k0, pi, A.pos, A.rho_old, A.cstart, S1, S2);
+            dens_epilogue(P, A, K, i, pi, S1, S2);
         }
         return;
     }
-    const BrickGeo g = brick_geo(P, S);
+    const BrickGeo g = brick_geo(P, S, (uint32_t)bm.base);
     const float* rho_old = A.rho_old;
-    bool ok = stage<S, false>(P, g, A.pos, A.cstart, sm, [&](Smem<S, false>& s, uint32_t slot, uint32_t jg, float4 pj) {
+    bool ok = stage<S, false>(P, k0, g, A.pos, A.cstart, sm, [&](Smem<S, false>& s, uint32_t slot, uint32_t jg, float4 pj) {
         s.p[slot] = make_float4(pj.x, pj.y, pj.z, 1.0f / fmaxf(rho_old[jg], 1e-3f));
     });
     if (!ok) {
-        for_each_owned_global<S>(P, g, A.pos, A.cstart, [&](int i, float4 pi) {
+        for_each_owned_global<S>(P, k0, g, A.pos, A.cstart, [&](int i, float4 pi) {
             float S1, S2;
-            dens_sum_global<S>(P, pi, A.pos, A.rho_old, A.cstart, S1, S2);
-            dens_epilogue(P, A, i, pi, S1, S2);
+            dens_sum_global<S>(P, k0, pi, A.pos, A.rho_old, A.cstart, S1, S2);
+            dens_epilogue(P, A, K, i, pi, S1, S2);
         });
         return;
     }
@@ -573,8 +589,8 @@ __global__ __launch_bounds__(BT) void k_density_eos_brick(DevParams P, DensArgs 
         int k = r + tid;
         bool active = k < nown;
         int i = active ? (int)sm.own[k] : 0;
-        if (g.wraps) dens_own<S, true>(P, A, g, sm, i, active);
-        else dens_own<S, false>(P, A, g, sm, i, active);
+        if (g.wraps) dens_own<S, true>(P, A, K, g, sm, i, active);
+        else dens_own<S, false>(P, A, K, g, sm, i, active);
     }
 }
 
@@ -650,35 +666,42 @@ __device__ __forceinline__ void reduce_maxv(uint32_t* maxv, float mx, float my, 
 }
 
 template <int S>
-__global__ __launch_bounds__(BT) void k_momentum_kdk_brick(DevParams P, MomArgs A) {
+__global__ __launch_bounds__(BT) void k_momentum_kdk_brick(const MemberDesc* __restrict__ mem,
+                                                           const BlockMap* __restrict__ bmap, MomArgs A) {
     __shared__ Smem<S, true> sm;
     const int tid = threadIdx.x;
+    const BlockMap bm = bmap[blockIdx.x];
+    const MemberDesc& MD = mem[bm.m];
+    const DevParams P = MD.P;
+    const uint32_t k0 = MD.k0;
+    uint32_t* maxv = A.maxv + bm.m * 192;
     float mx = 0.f, my = 0.f, mz = 0.f;
     prologue<S, true>(P, A.disp, A.disp_slots, sm);
     if (!sm.modeA) {
-        int i = blockIdx.x * BT + tid;
-        if (i < P.N) {
+        int il = bm.ord * BT + tid;
+        if (il < P.N) {
+            int i = MD.p0 + il;
             float4 pi = A.pos[i], vi = A.vel[i], ti = A.thq[i];
             float rhoi = fmaxf(A.rho[i], 1e-3f), pri = A.prs[i];
             MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            mom_sum_global<S>(P, pi, vi, rhoi, pri, A.pos, A.vel, A.rho, A.prs, A.cstart, M);
+            mom_sum_global<S>(P, k0, pi, vi, rhoi, pri, A.pos, A.vel, A.rho, A.prs, A.cstart, M);
             mom_epilogue(P, A, i, pi, vi, ti, pri, M, mx, my, mz);
         }
-        if (A.mode) reduce_maxv(A.maxv, mx, my, mz);
+        if (A.mode) reduce_maxv(maxv, mx, my, mz);
         return;
     }
-    const BrickGeo g = brick_geo(P, S);
+    const BrickGeo g = brick_geo(P, S, (uint32_t)bm.base);
     const float* rho = A.rho;
-    bool ok = stage<S, true>(P, g, A.pos, A.cstart, sm, [&](Smem<S, true>& s, uint32_t slot, uint32_t jg, float4 pj) {
+    bool ok = stage<S, true>(P, k0, g, A.pos, A.cstart, sm, [&](Smem<S, true>& s, uint32_t slot, uint32_t jg, float4 pj) {
         s.p[slot] = make_float4(pj.x, pj.y, pj.z, fmaxf(rho[jg], 1e-3f));
         s.j[slot] = jg;
     });
     if (!ok) {
-        for_each_owned_global<S>(P, g, A.pos, A.cstart, [&](int i, float4 pi) {
+        for_each_owned_global<S>(P, k0, g, A.pos, A.cstart, [&](int i, float4 pi) {
             float4 vi = A.vel[i], ti = A.thq[i];
             float rhoi = fmaxf(A.rho[i], 1e-3f), pri = A.prs[i];
             MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            mom_sum_global<S>(P, pi, vi, rhoi, pri, A.pos, A.vel, A.rho, A.prs, A.cstart, M);
+            mom_sum_global<S>(P, k0, pi, vi, rhoi, pri, A.pos, A.vel, A.rho, A.prs, A.cstart, M);
             mom_epilogue(P, A, i, pi, vi, ti, pri, M, mx, my, mz);
         });
     } else {
@@ -691,33 +714,30 @@ __global__ __launch_bounds__(BT) void k_momentum_kdk_brick(DevParams P, MomArgs 
             else mom_own<S, false>(P, A, g, sm, i, active, mx, my, mz);
         }
     }
-    if (A.mode) reduce_maxv(A.maxv, mx, my, mz);
+    if (A.mode) reduce_maxv(maxv, mx, my, mz);
 }
 
 }  // namespace
 
-int brick_count(const DevParams& P) {
-    uint32_t nkeys = 1u << P.end_bit;
-    return (int)std::max<uint32_t>(1u, nkeys >> NB);
-}
+int brick_bits() { return NB; }
 
-void launch_density_eos(const DevParams& P, hipStream_t s, const DensArgs& A) {
-    dim3 g(brick_count(P));
-    switch (P.S) {
-        case 1: k_density_eos_brick<1><<<g, BT, 0, s>>>(P, A); break;
-        case 2: k_density_eos_brick<2><<<g, BT, 0, s>>>(P, A); break;
-        case 3: k_density_eos_brick<3><<<g, BT, 0, s>>>(P, A); break;
-        default: k_density_eos_brick<4><<<g, BT, 0, s>>>(P, A); break;
+void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A) {
+    dim3 g(L.nbblk);
+    switch (S) {
+        case 1: k_density_eos_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
+        case 2: k_density_eos_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
+        case 3: k_density_eos_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
+        default: k_density_eos_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
     }
 }
 
-void launch_momentum_kdk(const DevParams& P, hipStream_t s, const MomArgs& A) {
-    dim3 g(brick_count(P));
-    switch (P.S) {
-        case 1: k_momentum_kdk_brick<1><<<g, BT, 0, s>>>(P, A); break;
-        case 2: k_momentum_kdk_brick<2><<<g, BT, 0, s>>>(P, A); break;
-        case 3: k_momentum_kdk_brick<3><<<g, BT, 0, s>>>(P, A); break;
-        default: k_momentum_kdk_brick<4><<<g, BT, 0, s>>>(P, A); break;
+void launch_momentum_kdk(const Launch& L, int S, hipStream_t s, const MomArgs& A) {
+    dim3 g(L.nbblk);
+    switch (S) {
+        case 1: k_momentum_kdk_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
+        case 2: k_momentum_kdk_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
+        case 3: k_momentum_kdk_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
+        default: k_momentum_kdk_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
     }
 }
 

@@ -1,14 +1,13 @@
 // asph_sort.hip - Morton keys and a stable LSD radix sort of (key, index) pairs for gfx950.
 //
-// Replaces kernel_morton_keys + kernel_iota + cub::DeviceRadixSort::SortPairs
-// (asph_h100.cu:229-244, 869, 903, 966): the result is the stable argsort of the keys over
-// bits [0, end_bit), i.e. ties keep the original index order.
+// Replaces kernel_iota + cub::DeviceRadixSort::SortPairs (asph_h100.cu:241-244, 869, 903, 966):
+// the result is the stable argsort of the keys over bits [0, end_bit), i.e. ties keep the
+// original index order. The keys come from k_keys (asph_kernels.hip).
 //
 // Each pass sorts one 8-bit digit (the last pass may be narrower) in three steps:
 //   1. k_radix_hist: each workgroup counts the digits of one tile of RS_TILE keys in LDS
 //      (wave-aggregated: lanes holding the same digit are found with 8 ballots and one lane
 //      adds the group's size), and stores its 256 counts digit-major in bh[d * nblk + b].
-//      The first pass computes the Morton keys from the positions on the fly.
 //   2. k_radix_scan_digits / k_radix_scan_totals: exclusive scan of bh in (digit, tile) order.
 //   3. k_radix_scatter: each workgroup ranks its tile stably in LDS -- rounds of 256 keys in
 //      index order; within a wave the rank is the popcount of the equal-digit lanes below,
@@ -36,15 +35,7 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, bool valid
     return m;
 }
 
-__device__ __forceinline__ uint32_t morton_of(const DevParams& P, float4 p) {
-    int ix, iy, iz;
-    cell_trunc(P, p.x, p.y, p.z, ix, iy, iz);
-    return morton3((uint32_t)ix, (uint32_t)iy, (uint32_t)iz);
-}
-
-__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(DevParams P, const float4* __restrict__ pos,
-                                                           const uint32_t* __restrict__ kin,
-                                                           uint32_t* __restrict__ kout0, int N,
+__global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __restrict__ kin, int N,
                                                            int shift, int bits,
                                                            uint32_t* __restrict__ bh, int nblk) {
     __shared__ uint32_t h[256];
@@ -57,15 +48,7 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(DevParams P, const fl
     for (int r = 0; r < RS_ITEMS; ++r) {
         int idx = base + r * RS_THREADS + t;
         bool valid = idx < N;
-        uint32_t k = 0;
-        if (valid) {
-            if (pos) {
-                k = morton_of(P, pos[idx]);
-                kout0[idx] = k;
-            } else {
-                k = kin[idx];
-            }
-        }
+        uint32_t k = valid ? kin[idx] : 0u;
         uint32_t d = (k >> shift) & mask;
         uint64_t m = match_digit(d, bits, valid);
         if (valid && lane == __ffsll((unsigned long long)m) - 1) atomicAdd(&h[d], (uint32_t)__popcll(m));
@@ -205,21 +188,19 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
 
 int radix_tile() { return RS_TILE; }
 
-void launch_radix_sort(const DevParams& P, hipStream_t s, const float4* pos, uint32_t* keys0,
-                       uint32_t* kA, uint32_t* vA, uint32_t* kB, uint32_t* vB, uint32_t* bh,
-                       uint32_t* bo, uint32_t* dtot, uint32_t** keys_sorted, uint32_t** perm) {
-    const int N = P.N;
+void launch_radix_sort(hipStream_t s, int N, int end_bit, const uint32_t* keys0, uint32_t* kA,
+                       uint32_t* vA, uint32_t* kB, uint32_t* vB, uint32_t* bh, uint32_t* bo,
+                       uint32_t* dtot, uint32_t** keys_sorted, uint32_t** perm) {
     const int nblk = (N + RS_TILE - 1) / RS_TILE;
-    const int npass = (P.end_bit + 7) / 8;
+    const int npass = (end_bit + 7) / 8;
     const uint32_t* kin = keys0;
     const uint32_t* vin = nullptr;
     uint32_t* ko = kA;
     uint32_t* vo = vA;
     for (int p = 0; p < npass; ++p) {
         int shift = 8 * p;
-        int bits = min(8, P.end_bit - shift);
-        k_radix_hist<<<nblk, RS_THREADS, 0, s>>>(P, p == 0 ? pos : nullptr, kin, keys0, N, shift,
-                                                 bits, bh, nblk);
+        int bits = min(8, end_bit - shift);
+        k_radix_hist<<<nblk, RS_THREADS, 0, s>>>(kin, N, shift, bits, bh, nblk);
         k_radix_scan_digits<<<256, RS_THREADS, 0, s>>>(bh, bo, dtot, nblk);
         k_radix_scan_totals<<<1, RS_THREADS, 0, s>>>(dtot);
         k_radix_scatter<<<nblk, RS_THREADS, 0, s>>>(kin, vin, ko, vo, N, shift, bits, bh, bo, dtot,

@@ -63,13 +63,44 @@ def ic_generate(cfg: L.Config, ic: L.ICSpec) -> State:
 
 
 class Sim:
-    def __init__(self, cfg: L.Config, stream: int | None = None):
+    """One device handle: a single domain (cfg) or an ensemble (a list of configs, members
+    concatenated in every state array; see member_range)."""
+
+    def __init__(self, cfg, stream: int | None = None):
         self.lib = L.load()
-        self.cfg = cfg.copy()
         self.h = C.c_void_p()
-        L.check(self.lib.asph_create(C.byref(self.cfg), C.c_void_p(stream or 0), C.byref(self.h)),
-                "asph_create")
+        if isinstance(cfg, (list, tuple)):
+            arr = (L.Config * len(cfg))(*[c.copy() for c in cfg])
+            self.cfgs = [c.copy() for c in cfg]
+            L.check(self.lib.asph_ensemble_create(arr, len(cfg), C.c_void_p(stream or 0), C.byref(self.h)),
+                    "asph_ensemble_create")
+        else:
+            self.cfgs = [cfg.copy()]
+            L.check(self.lib.asph_create(C.byref(self.cfgs[0]), C.c_void_p(stream or 0), C.byref(self.h)),
+                    "asph_create")
+        self.cfg = self.cfgs[0]
         self.n = int(self.lib.asph_num_particles(self.h))
+        self.ncol = sum(c.nx * c.ny for c in self.cfgs)
+
+    @property
+    def members(self) -> int:
+        return int(self.lib.asph_member_count(self.h))
+
+    def member_range(self, m: int):
+        a, n = C.c_int64(), C.c_int64()
+        L.check(self.lib.asph_member_range(self.h, m, C.byref(a), C.byref(n)), "asph_member_range")
+        return int(a.value), int(n.value)
+
+    def member_max_vel(self, m: int) -> np.ndarray:
+        out = np.empty(3, np.float32)
+        L.check(self.lib.asph_member_max_vel(self.h, m, _fp(out)), "asph_member_max_vel")
+        return out
+
+    def run_graph(self, nsteps: int, dt: float, sort_every: int = 0):
+        """nsteps steps replayed from captured hipGraphs (no host sync between steps)."""
+        st = L.RunStats()
+        L.check(self.lib.asph_run_graph(self.h, nsteps, dt, sort_every, C.byref(st)), "asph_run_graph")
+        return st
 
     def close(self):
         if self.h:
@@ -114,14 +145,14 @@ class Sim:
         L.check(self.lib.asph_set_acc(self.h, _fp(a)), "asph_set_acc")
 
     def get_rain_fx(self):
-        ncell, ncol = self.n, self.cfg.nx * self.cfg.ny
+        ncell, ncol = self.n, self.ncol
         r, p, c = np.empty(ncell, np.int64), np.empty(ncol, np.int64), C.c_int64()
         L.check(self.lib.asph_get_rain_fx(self.h, r.ctypes.data_as(L.I64P), p.ctypes.data_as(L.I64P),
                                           C.byref(c)), "asph_get_rain_fx")
         return r, p, int(c.value)
 
     def get_rain(self):
-        ncell, ncol = self.n, self.cfg.nx * self.cfg.ny
+        ncell, ncol = self.n, self.ncol
         r, p = np.empty(ncell, np.float32), np.empty(ncol, np.float32)
         L.check(self.lib.asph_get_rain(self.h, _fp(r), _fp(p)), "asph_get_rain")
         return r, p
@@ -138,7 +169,7 @@ class Sim:
         n, ncell = self.n, self.n
         keys, ks, perm = (np.empty(n, np.uint32) for _ in range(3))
         cr = np.empty(2 * ncell, np.int32)
-        sl = np.empty(2 * self.cfg.nx * self.cfg.ny, np.int32)
+        sl = np.empty(2 * self.ncol, np.int32)
         sc = C.c_int32()
         L.check(self.lib.asph_get_sort(self.h, keys.ctypes.data_as(L.U32P), ks.ctypes.data_as(L.U32P),
                                        perm.ctypes.data_as(L.U32P), cr.ctypes.data_as(L.I32P),
