@@ -56,6 +56,28 @@ def cpu_baseline(cfg, st, dt, steps):
                        f"FP32 oracle, OpenMP, {wall:.1f} s")
 
 
+def copy_probe(mib):
+    """Three plain device-to-device copies of `mib` MiB through the HIP runtime's blit kernel: a
+    kernel of known traffic (reads and writes mib MiB each) against which FETCH_SIZE and
+    WRITE_SIZE counters are scaled."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    n = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    ok = hip.hipMalloc(C.byref(a), n) == 0 and hip.hipMalloc(C.byref(b), n) == 0
+    ok = ok and hip.hipMemset(a, 1, n) == 0 and hip.hipDeviceSynchronize() == 0
+    for _ in range(3):
+        ok = ok and hip.hipMemcpy(b, a, n, 3) == 0          # 3 = hipMemcpyDeviceToDevice
+    ok = ok and hip.hipDeviceSynchronize() == 0
+    for ptr in (a, b):
+        if ptr.value:
+            hip.hipFree(ptr)
+    if not ok:
+        raise SystemExit("copy probe: HIP runtime call failed")
+    log(f"[bench] copy probe: 3 x {mib} MiB device-to-device")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -65,6 +87,9 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override lattice edge (cubic)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this size 3 times (device to device) as "
+                         "the byte reference for hardware traffic counters")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the step is a single-GPU kernel set "
@@ -141,6 +166,8 @@ def main():
         },
         "cpu_baseline": None,
     }
+    if args.copy_probe_mib:
+        copy_probe(args.copy_probe_mib)
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle sample ...")
         out["cpu_baseline"] = cpu_baseline(cfg, st, dt, args.cpu_steps)

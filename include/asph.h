@@ -288,6 +288,29 @@ int asph_set_profiling(asph_t* h, int32_t enable);
 int asph_kernel_times(asph_t* h, double ms[ASPH_K_COUNT], int64_t counts[ASPH_K_COUNT]);
 int asph_synchronize(asph_t* h);
 
+/* Work counters of the two neighbour-loop kernels (no reference counterpart; diagnostics of
+ * which path each brick took and how much pair work it did). Indices into the array that
+ * asph_get_counters fills, summed over launches since the last read: */
+#define ASPH_C_DENS_ONE_SLAB  0  /* density: bricks whose halo was staged in LDS at once      */
+#define ASPH_C_DENS_SLABBED   1  /*          bricks whose halo was streamed in z-slabs        */
+#define ASPH_C_DENS_STAGINGS  2  /*          slab stagings (slabs x rounds of 512 particles) */
+#define ASPH_C_DENS_GLOBAL    3  /*          bricks that took the global-memory loop          */
+#define ASPH_C_DENS_MODEB     4  /*          workgroups of the thread-per-particle mode       */
+#define ASPH_C_DENS_CAND      5  /*          candidates tested after culling (LDS paths)      */
+#define ASPH_C_DENS_PAIRS     6  /*          in-support pairs evaluated (LDS paths)           */
+#define ASPH_C_MOM_ONE_SLAB   7  /* momentum: as 0..6                                          */
+#define ASPH_C_MOM_SLABBED    8
+#define ASPH_C_MOM_STAGINGS   9
+#define ASPH_C_MOM_GLOBAL    10
+#define ASPH_C_MOM_MODEB     11
+#define ASPH_C_MOM_CAND      12
+#define ASPH_C_MOM_PAIRS     13
+#define ASPH_C_COUNT         16
+/* Enables (allocating and zeroing the counters) or disables counting. */
+int asph_set_counters(asph_t* h, int32_t enable);
+/* Synchronises, copies the counters to out[ASPH_C_COUNT] and zeroes them. */
+int asph_get_counters(asph_t* h, uint64_t out[ASPH_C_COUNT]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -25,6 +25,10 @@ F_SORT_EVERY_STEP = 1 << 8
 QV_DRY, QV_REF_EXP, QV_RH_TABLE = 0, 1, 2
 
 K_KICK_DRIFT, K_DENSITY_EOS, K_MOMENTUM_KDK, K_RAIN, K_SORT = range(5)
+C_COUNT = 16
+COUNTER_NAMES = ("dens_one_slab", "dens_slabbed", "dens_stagings", "dens_global", "dens_modeb",
+                 "dens_cand", "dens_pairs", "mom_one_slab", "mom_slabbed", "mom_stagings",
+                 "mom_global", "mom_modeb", "mom_cand", "mom_pairs")
 K_COUNT = 5
 KERNEL_NAMES = ("kick_drift", "density_eos", "momentum_kdk", "rain", "sort")
 
@@ -143,6 +147,8 @@ SIGNATURES = {
     "asph_set_profiling": (C.c_int, [P, C.c_int32]),
     "asph_kernel_times": (C.c_int, [P, DP, I64P]),
     "asph_synchronize": (C.c_int, [P]),
+    "asph_set_counters": (C.c_int, [P, C.c_int32]),
+    "asph_get_counters": (C.c_int, [P, C.POINTER(C.c_uint64)]),
 }
 
 _LIB = None

@@ -54,6 +54,8 @@ struct asph_t {
     std::map<std::tuple<void*, void*, void*, void*, void*, void*, int, float>, hipGraphExec_t> graphs;
     // profiling
     bool prof = false;
+    unsigned long long* stats = nullptr;   // work counters of the neighbour kernels (enabled)
+    unsigned long long* stats_buf = nullptr;
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_live[ASPH_K_COUNT];
 
@@ -201,6 +203,7 @@ MomArgs mom_args(asph_t* h, float dt, int mode) {
     A.pos = h->pos; A.vel = h->vel; A.thq = h->thq; A.rho = h->rho; A.prs = h->prs;
     A.cstart = h->cstart; A.acc = h->acc; A.dthdt = h->dthdt; A.vel_out = h->vel2;
     A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
+    A.stats = h->stats;
     A.dt = dt; A.mode = mode;
     return A;
 }
@@ -245,6 +248,7 @@ void do_density_eos(asph_t* h, float dt, double simt, int bits, bool use_clock, 
         A.vel = h->vel; A.cstart = h->cstart; A.keys_sorted = h->keys_sorted; A.rain = h->rain;
         A.dep = h->dep; A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
         A.clock = use_clock ? h->clock : nullptr; A.simt = simt;
+        A.stats = h->stats;
         A.dt = dt; A.bits = bits;
         ProfScope ps(h, ASPH_K_DENSITY_EOS);
         launch_density_eos(h->L, h->S, h->stream, A);
@@ -843,6 +847,44 @@ int asph_kernel_times(asph_t* h, double ms[ASPH_K_COUNT], int64_t counts[ASPH_K_
         if (ms) ms[k] = tot;
         if (counts) counts[k] = (int64_t)h->ev_live[k].size();
         h->ev_live[k].clear();
+    }
+    return ASPH_OK;
+}
+
+int asph_set_counters(asph_t* h, int32_t enable) {
+    if (!h) return fail("null handle");
+    if (enable && !h->stats_buf) {
+        int rc = h->alloc(&h->stats_buf, (size_t)ASPH_C_COUNT * 64);   // 64 slots per counter
+        if (rc) return rc;
+    }
+    unsigned long long* want = enable ? h->stats_buf : nullptr;
+    if (want != h->stats) {
+        // captured graphs hold the counter pointer as a kernel argument
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+        h->graphs.clear();
+    }
+    if (enable)
+        HIPCHK(hipMemsetAsync(h->stats_buf, 0, ASPH_C_COUNT * 64 * sizeof(unsigned long long), h->stream));
+    h->stats = want;
+    return ASPH_OK;
+}
+
+int asph_get_counters(asph_t* h, uint64_t out[ASPH_C_COUNT]) {
+    if (!h || !out) return fail("null argument");
+    if (!h->stats_buf) {
+        for (int k = 0; k < ASPH_C_COUNT; ++k) out[k] = 0;
+        return ASPH_OK;
+    }
+    std::vector<uint64_t> slots((size_t)ASPH_C_COUNT * 64);
+    HIPCHK(hipMemcpyAsync(slots.data(), h->stats_buf, slots.size() * sizeof(uint64_t),
+                          hipMemcpyDeviceToHost, h->stream));
+    HIPCHK(hipMemsetAsync(h->stats_buf, 0, slots.size() * sizeof(uint64_t), h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    for (int k = 0; k < ASPH_C_COUNT; ++k) {
+        uint64_t t = 0;
+        for (int j = 0; j < 64; ++j) t += slots[(size_t)k * 64 + j];
+        out[k] = t;
     }
     return ASPH_OK;
 }

@@ -75,15 +75,29 @@ ASPH_DH float dqsdT(float T, float es, float p) {
     float den = p - 0.378f * es;
     return ((0.622f * p) * desdT) / (den * den);
 }
+// Piecewise-linear RH(z) over non-decreasing knots, clamped outside. The segment is the last
+// k <= n-2 with z >= rh_z[k] (where the scan k = 0, 1, ... while z >= rh_z[k+1] stops), picked
+// with constant indices only so the table can live in registers.
 ASPH_DH float rh_interp(const DevParams& P, float z) {
-    int n = P.rh_n;
+    const int n = P.rh_n;
     if (n <= 0) return 0.0f;
     if (z <= P.rh_z[0]) return P.rh_v[0];
-    if (z >= P.rh_z[n - 1]) return P.rh_v[n - 1];
-    int k = 0;
-    while (z >= P.rh_z[k + 1]) ++k;
-    float w = (z - P.rh_z[k]) / (P.rh_z[k + 1] - P.rh_z[k]);
-    return P.rh_v[k] + (P.rh_v[k + 1] - P.rh_v[k]) * w;
+    float zl = P.rh_z[0], vl = P.rh_v[0];
+#pragma unroll
+    for (int k = 1; k < ASPH_RH_KNOTS; ++k) {
+        if (k == n - 1) { zl = P.rh_z[k]; vl = P.rh_v[k]; }
+    }
+    if (z >= zl) return vl;
+    float za = P.rh_z[0], va = P.rh_v[0], zb = P.rh_z[1], vb = P.rh_v[1];
+#pragma unroll
+    for (int k = 1; k + 1 < ASPH_RH_KNOTS; ++k) {
+        if (k + 1 < n && z >= P.rh_z[k]) {
+            za = P.rh_z[k]; va = P.rh_v[k];
+            zb = P.rh_z[k + 1]; vb = P.rh_v[k + 1];
+        }
+    }
+    float w = (z - za) / (zb - za);
+    return va + (vb - va) * w;
 }
 // Background moisture; ASPH_QV_REF_EXP is asph_h100.cu:149-153.
 ASPH_DH float qv_ref(const DevParams& P, float z) {

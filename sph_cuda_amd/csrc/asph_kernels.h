@@ -32,6 +32,7 @@ struct DensArgs {
     const uint32_t* disp_slots;
     const double* clock;         // simulated time at the start of the step (device), or null:
     double simt;                 // then this value is used
+    unsigned long long* stats;   // work counters (ASPH_C_DENS_*), or null
     float dt;
     int bits;                    // bit 0: EOS + microphysics, bit 1: surface flux (if flagged)
 };
@@ -49,6 +50,7 @@ struct MomArgs {
     uint32_t* maxv;              // [n_members][3][64]
     const float* disp;
     const uint32_t* disp_slots;
+    unsigned long long* stats;   // work counters (ASPH_C_MOM_*), or null
     float dt;
     int mode;                    // 0: force evaluation, 1: step
 };

@@ -23,20 +23,29 @@ constexpr int TPB = 256;
 // ---------------------------------------------------------------------------------------------
 // kick(dt/2) + drift, asph_h100.cu:535-561. Also clears the per-step rain deposit grid
 // (consumed by k_rain of the previous step) and records the step's drift bound.
+//
+// Grid-stride over the particle blocks with at most KD_GRID workgroups: each workgroup reduces
+// its drift maxima in registers and LDS and issues three atomics at the end, so the 64 slots per
+// axis see a few dozen atomics each (one atomic per wavefront serialised ~4k device-scope
+// atomics on every slot at 256^3).
 // ---------------------------------------------------------------------------------------------
+constexpr int KD_GRID = 2048;
+
 __global__ __launch_bounds__(TPB) void k_kick_drift(const MemberDesc* __restrict__ mem,
-                                                    const BlockMap* __restrict__ pmap,
+                                                    const BlockMap* __restrict__ pmap, int npblk,
                                                     float4* __restrict__ pos, float4* __restrict__ vel,
                                                     const float4* __restrict__ acc,
                                                     int64_t* __restrict__ dep,
                                                     uint32_t* __restrict__ disp_slots, float hdt,
                                                     float dt) {
-    const BlockMap bm = pmap[blockIdx.x];
-    const MemberDesc& M = mem[bm.m];
-    const DevParams& P = M.P;
-    const int il = bm.base + threadIdx.x;
+    __shared__ float red[3][TPB / 64];
     float mx = 0.f, my = 0.f, mz = 0.f;
-    if (il < P.N) {
+    for (int b = blockIdx.x; b < npblk; b += gridDim.x) {
+        const BlockMap bm = pmap[b];
+        const MemberDesc& M = mem[bm.m];
+        const DevParams& P = M.P;
+        const int il = bm.base + threadIdx.x;
+        if (il >= P.N) continue;
         const int i = M.p0 + il;
         float4 v = vel[i];
         float4 a = acc[i];
@@ -58,9 +67,9 @@ __global__ __launch_bounds__(TPB) void k_kick_drift(const MemberDesc* __restrict
         vel[i] = v;
         dep[i] = 0;   // cells = particles per member: cell index = particle index
         // drift bound of this step in cells per axis (|dx| <= |v dt|, also across a wrap or a wall)
-        mx = fabsf(v.x * dt) * P.inv_dx;
-        my = fabsf(v.y * dt) * P.inv_dy;
-        mz = fabsf(v.z * dt) * P.inv_dz;
+        mx = fmaxf(mx, fabsf(v.x * dt) * P.inv_dx);
+        my = fmaxf(my, fabsf(v.y * dt) * P.inv_dy);
+        mz = fmaxf(mz, fabsf(v.z * dt) * P.inv_dz);
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -68,11 +77,15 @@ __global__ __launch_bounds__(TPB) void k_kick_drift(const MemberDesc* __restrict
         my = fmaxf(my, __shfl_xor(my, o));
         mz = fmaxf(mz, __shfl_xor(mz, o));
     }
-    if ((threadIdx.x & 63) == 0) {
-        int slot = (blockIdx.x * (TPB / 64) + (threadIdx.x >> 6)) & 63;
-        atomicMax(&disp_slots[slot], __float_as_uint(mx));
-        atomicMax(&disp_slots[64 + slot], __float_as_uint(my));
-        atomicMax(&disp_slots[128 + slot], __float_as_uint(mz));
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) { red[0][w] = mx; red[1][w] = my; red[2][w] = mz; }
+    __syncthreads();
+    if (threadIdx.x < 3) {
+        float m = red[threadIdx.x][0];
+#pragma unroll
+        for (int k = 1; k < TPB / 64; ++k) m = fmaxf(m, red[threadIdx.x][k]);
+        // non-negative floats order like their bit patterns
+        atomicMax(&disp_slots[threadIdx.x * 64 + (blockIdx.x & 63)], __float_as_uint(m));
     }
 }
 
@@ -336,7 +349,9 @@ void launch_step_begin(hipStream_t s, float* disp, uint32_t* disp_slots, int res
 
 void launch_kick_drift(const Launch& L, hipStream_t s, float4* pos, float4* vel, const float4* acc,
                        int64_t* dep, uint32_t* disp_slots, float dt) {
-    k_kick_drift<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, pos, vel, acc, dep, disp_slots, 0.5f * dt, dt);
+    const int grid = L.npblk < KD_GRID ? L.npblk : KD_GRID;
+    k_kick_drift<<<grid, TPB, 0, s>>>(L.mem, L.pmap, L.npblk, pos, vel, acc, dep, disp_slots,
+                                      0.5f * dt, dt);
 }
 
 void launch_rain(const Launch& L, hipStream_t s, const int64_t* rain_in, const int64_t* dep,

@@ -4,13 +4,25 @@
 //
 // Work decomposition (mode A): one 512-thread workgroup per 8x8x8-cell brick of a member's
 // Morton key space (local keys [base, base+512), listed in the brick map; an ensemble launches
-// the bricks of all its members together). The workgroup owns the particles whose CURRENT cell lies in
-// its brick and stages, into LDS, every particle LISTED (by the last sort) in the brick's halo
-// (brick +- stencil cells), stored in row-major order of the halo cells with a per-cell start
-// table. For an owned particle i each stencil row (jz, jy) is then one contiguous LDS span, and
-// walking rows jz -> jy and spans left to right visits candidates exactly in the reference's
-// jz -> jy -> jx -> j order (asph_h100.cu:306-327, 444-495): the sums are bit-identical to a
-// thread-per-particle loop over the cell table.
+// the bricks of all its members together). The workgroup owns the particles LISTED (by the last
+// sort) in its brick's cells: in key order these are one contiguous index range, taken in rounds
+// of 512 with one particle per thread, so a brick of any occupancy needs no owned-particle list
+// and neighbouring lanes hold neighbouring particles. It stages into LDS every particle listed in
+// the brick's halo (brick +- margin cells), stored in row-major order of the halo cells with a
+// per-cell start table. The margin is the stencil S when the drift bound d since the sort allows
+// (a particle lies within [c - d, c + 1 + d] of its listed cell c, so the culled walk of its
+// current cell's stencil stays within c +- floor(1 + 2d + R), R the culling radius), else S + 1.
+// For a particle each stencil row (jz, jy) is one contiguous LDS span, and walking rows jz -> jy
+// and spans left to right visits candidates exactly in the reference's jz -> jy -> jx -> j order
+// (asph_h100.cu:306-327, 444-495): the sums are bit-identical to a thread-per-particle loop over
+// the cell table.
+//
+// A halo holding more than MAXP particles (clumped or compressed regions) is streamed through
+// LDS in slabs of whole z-layers: each thread keeps its sums in registers and walks, slab by
+// slab, the rows of its stencil that lie in the staged layers, so the candidate order is still
+// the reference's. Only a brick with a single halo layer beyond MAXP, or more than 65534 halo
+// particles, runs the thread-per-particle loop over the global cell table, as does mode B
+// (drift since the last sort >= 0.95 cell, or a domain narrower than a halo).
 //
 // Candidate culling is exact: particles listed in cell c have moved at most delta cells (the
 // device-tracked maximum drift since the sort) so they lie within [c - delta, c + 1 + delta] on
@@ -22,10 +34,6 @@
 // slots (QK entries): the test loop appends, and when any lane of the wavefront has a full queue
 // all lanes drain theirs in order. Lanes therefore spend the expensive pair body only on
 // in-support pairs, in the reference order.
-//
-// Mode B (drift since the last sort >= 0.95 cell, or a domain narrower than a halo) and the
-// overflow fallback of a brick (more than MAXP halo or MAXOWN owned particles) run the
-// thread-per-particle loop over the global cell table, with the same arithmetic.
 #include "asph_kernels.h"
 
 namespace asph {
@@ -36,15 +44,9 @@ constexpr int NB = 9;              // Morton bits per brick: 8 x 8 x 8 cells
 constexpr int BS = 8;              // brick edge in cells
 constexpr int BT = 512;            // threads per workgroup (one per brick cell)
 constexpr int BW = BT / 64;        // wavefronts per workgroup
-constexpr int MAXP = 3072;         // halo particles held in LDS
-constexpr int MAXOWN = 768;        // owned particles per brick
+constexpr int MAXP = 3072;         // halo particles held in LDS at a time
 constexpr int QK = 8;              // queued in-support pairs per thread
 constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cells]
-
-template <int S> struct Halo {
-    static constexpr int H = BS + 2 * S;
-    static constexpr int HC = H * H * H;
-};
 
 __device__ __forceinline__ void min_image(const DevParams& P, float& rx, float& ry) {
     if (P.flags & ASPH_F_MIN_IMAGE) {
@@ -66,7 +68,7 @@ template <int S> __device__ __forceinline__ void zrange(const DevParams& P, int 
 // Thread-per-particle loops over the global cell table (mode B and overflow fallback).
 // ---------------------------------------------------------------------------------------------
 template <int S>
-__device__ void dens_sum_global(const DevParams& P, uint32_t k0, float4 pi, const float4* __restrict__ pos,
+__device__ __forceinline__ void dens_sum_global(const DevParams& P, uint32_t k0, float4 pi, const float4* __restrict__ pos,
                                 const float* __restrict__ rho_old, const uint32_t* __restrict__ cstart,
                                 float& S1, float& S2) {
     int ix, iy, iz, z0, z1;
@@ -132,7 +134,7 @@ __device__ __forceinline__ void mom_pair(const DevParams& P, float rx, float ry,
 }
 
 template <int S>
-__device__ void mom_sum_global(const DevParams& P, uint32_t k0, float4 pi, float4 vi, float rhoi, float pri,
+__device__ __forceinline__ void mom_sum_global(const DevParams& P, uint32_t k0, float4 pi, float4 vi, float rhoi, float pri,
                                const float4* __restrict__ pos, const float4* __restrict__ vel,
                                const float* __restrict__ rho, const float* __restrict__ prs,
                                const uint32_t* __restrict__ cstart, MomAcc& A) {
@@ -174,7 +176,7 @@ struct DensConst {
     int p0;          // member's first particle (= first cell)
 };
 
-__device__ void dens_epilogue(const DevParams& P, const DensArgs& A, const DensConst& K, int i,
+__device__ __forceinline__ void dens_epilogue(const DevParams& P, const DensArgs& A, const DensConst& K, int i,
                               float4 pi, float S1, float S2) {
     const float z = pi.z;
     const float ex = exner_ref(P, z);
@@ -266,7 +268,7 @@ __device__ void dens_epilogue(const DevParams& P, const DensArgs& A, const DensC
 }
 
 // momentum finish (:497-510) + theta update, second kick, sponge (mode 1). Returns |v| maxima.
-__device__ void mom_epilogue(const DevParams& P, const MomArgs& A, int i, float4 pi, float4 vi,
+__device__ __forceinline__ void mom_epilogue(const DevParams& P, const MomArgs& A, int i, float4 pi, float4 vi,
                              float4 ti, float pri, MomAcc M, float& mx, float& my, float& mz) {
     float ax = M.ax * P.mm, ay = M.ay * P.mm, az = M.az * P.mm;
     float avx = M.avx * P.mm, avy = M.avy * P.mm, avz = M.avz * P.mm;
@@ -307,38 +309,38 @@ __device__ void mom_epilogue(const DevParams& P, const MomArgs& A, int i, float4
 // ---------------------------------------------------------------------------------------------
 // Brick staging shared by both kernels.
 // ---------------------------------------------------------------------------------------------
+template <int S> struct HaloMax {
+    static constexpr int H = BS + 2 * (S + 1);
+    static constexpr int HC = H * H * H;
+};
+
 template <int S, bool MOM> struct Smem {
-    float4 p[MAXP];                          // x, y, z, and 1/max(rho_old,1e-3) or max(rho,1e-3)
-    uint32_t j[MOM ? MAXP : 1];              // global index of each staged particle (momentum)
-    uint16_t off[Halo<S>::HC + 1];           // exclusive start of each halo cell, row-major
-    uint32_t own[MAXOWN];                    // owned particles (global indices)
-    uint16_t q[QK][BT];                      // per-thread queue of LDS slots
+    float4 p[MAXP];                    // x, y, z, and 1/max(rho_old,1e-3) or max(rho,1e-3)
+    uint32_t j[MOM ? MAXP : 1];        // global index of each staged particle (momentum)
+    uint16_t off[HaloMax<S>::HC + 1];  // exclusive start of each halo cell over the whole halo
+    uint16_t q[QK][BT];                // per-thread queue of LDS slots
     uint32_t wsum[BW];
-    int nown;
-    int total;
+    uint8_t slab[HaloMax<S>::H + 1];   // slab boundaries in halo z-layers
+    int nslab;                         // 0: the halo cannot be staged (global-memory loop)
     int modeA;
+    int mrg[3];                        // halo margin per axis [cells]
     float delta[3];
 };
 
 struct BrickGeo {
-    int bx0, by0, bz0;
-    bool wraps;  // the halo crosses a periodic face
+    int bx0, by0, bz0;  // brick origin [cells]
+    int hx0, hy0, hz0;  // halo origin
+    int Hx, Hy, Hz;     // halo extent
+    bool wraps;         // the halo crosses a periodic face
 };
 
-__device__ __forceinline__ BrickGeo brick_geo(const DevParams& P, int S, uint32_t kb) {
-    BrickGeo g;
-    g.bx0 = (int)compact1by2(kb);
-    g.by0 = (int)compact1by2(kb >> 1);
-    g.bz0 = (int)compact1by2(kb >> 2);
-    const int H = BS + 2 * S;
-    g.wraps = (g.bx0 - S < 0) || (g.bx0 - S + H > P.nx) || (g.by0 - S < 0) || (g.by0 - S + H > P.ny);
-    return g;
-}
-
-// Reads the drift bound since the last sort (cells) and decides between mode A and mode B.
+// Reads the drift bound since the last sort and decides between mode A and mode B; in mode A
+// also the halo margins. A particle listed in cell c lies within [c - d, c + 1 + d] (d the drift
+// bound), its current cell within c +- 1, and the cells the culled walk visits within
+// c +- min(S + 1, floor(1 + 2d + R)), R the culling radius: the margin per axis.
 template <int S, bool MOM>
-__device__ void prologue(const DevParams& P, const float* disp, const uint32_t* disp_slots,
-                         Smem<S, MOM>& sm) {
+__device__ __forceinline__ void prologue(const DevParams& P, const float* disp, const uint32_t* disp_slots,
+                                         Smem<S, MOM>& sm) {
     const int tid = threadIdx.x;
     if (tid < 64) {
         float m[3];
@@ -350,50 +352,67 @@ __device__ void prologue(const DevParams& P, const float* disp, const uint32_t* 
             m[a] = v;
         }
         if (tid == 0) {
-            bool ok = P.nx >= Halo<S>::H && P.ny >= Halo<S>::H;
+            const float R = sqrtf(P.lim * 1.0001f) + DELTA_EPS;
+            bool ok = true;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 float d = disp[a] + m[a];
                 sm.delta[a] = d + DELTA_EPS;
                 ok = ok && (d < 0.95f);
+                int mg = (int)floorf(1.0f + 2.0f * sm.delta[a] + R);
+                sm.mrg[a] = mg < S + 1 ? mg : S + 1;
             }
+            ok = ok && P.nx >= BS + 2 * sm.mrg[0] && P.ny >= BS + 2 * sm.mrg[1];
             sm.modeA = ok ? 1 : 0;
-            sm.nown = 0;
         }
     }
     __syncthreads();
 }
 
-// Stages the halo of this workgroup's brick; returns false if it does not fit.
-template <int S, bool MOM, class Rec>
-__device__ bool stage(const DevParams& P, uint32_t k0, const BrickGeo& g, const float4* __restrict__ pos,
-                      const uint32_t* __restrict__ cstart, Smem<S, MOM>& sm, Rec rec) {
-    constexpr int H = Halo<S>::H;
-    constexpr int HC = Halo<S>::HC;
-    constexpr int PER = (HC + BT - 1) / BT;
+template <int S, bool MOM>
+__device__ __forceinline__ BrickGeo brick_geo(const DevParams& P, const Smem<S, MOM>& sm, uint32_t kb) {
+    BrickGeo g;
+    g.bx0 = (int)compact1by2(kb);
+    g.by0 = (int)compact1by2(kb >> 1);
+    g.bz0 = (int)compact1by2(kb >> 2);
+    g.Hx = BS + 2 * sm.mrg[0];
+    g.Hy = BS + 2 * sm.mrg[1];
+    g.Hz = BS + 2 * sm.mrg[2];
+    g.hx0 = g.bx0 - sm.mrg[0];
+    g.hy0 = g.by0 - sm.mrg[1];
+    g.hz0 = g.bz0 - sm.mrg[2];
+    g.wraps = (g.hx0 < 0) || (g.hx0 + g.Hx > P.nx) || (g.hy0 < 0) || (g.hy0 + g.Hy > P.ny);
+    return g;
+}
+
+__device__ __forceinline__ uint32_t halo_key(const DevParams& P, uint32_t k0, const BrickGeo& g, int c,
+                                             bool& inside) {
+    const int hx = c % g.Hx, hy = (c / g.Hx) % g.Hy, hz = c / (g.Hx * g.Hy);
+    const int gz = g.hz0 + hz;
+    inside = gz >= 0 && gz < P.nz;
+    return k0 | morton3((uint32_t)wrapi(g.hx0 + hx, P.nx), (uint32_t)wrapi(g.hy0 + hy, P.ny),
+                        (uint32_t)(inside ? gz : 0));
+}
+
+// Counts the halo's particles per cell, turns the counts into start offsets over the whole
+// halo, and splits the halo into slabs of whole z-layers holding at most MAXP particles each.
+// Sets sm.nslab = 0 if that is impossible (16-bit offsets or one layer beyond MAXP).
+template <int S, bool MOM>
+__device__ __forceinline__ void plan(const DevParams& P, uint32_t k0, const BrickGeo& g,
+                                     const uint32_t* __restrict__ cstart, Smem<S, MOM>& sm) {
     const int tid = threadIdx.x;
-    const int hx0 = g.bx0 - S, hy0 = g.by0 - S, hz0 = g.bz0 - S;
-    // counts of PER consecutive halo cells per thread, then a block-wide exclusive scan
-    uint32_t cnt[PER];
-    uint32_t start[PER];
+    const int HC = g.Hx * g.Hy * g.Hz;
+    const int PER = (HC + BT - 1) / BT;
+    const int c0 = tid * PER;
     uint32_t local = 0;
-#pragma unroll
     for (int k = 0; k < PER; ++k) {
-        int c = tid * PER + k;
-        cnt[k] = 0;
-        start[k] = 0;
-        if (c < HC) {
-            int hx = c % H, hy = (c / H) % H, hz = c / (H * H);
-            int gz = hz0 + hz;
-            if (gz >= 0 && gz < P.nz) {
-                uint32_t key = k0 | morton3((uint32_t)wrapi(hx0 + hx, P.nx), (uint32_t)wrapi(hy0 + hy, P.ny),
-                                            (uint32_t)gz);
-                uint32_t s = cstart[key];
-                start[k] = s;
-                cnt[k] = cstart[key + 1] - s;
-            }
-        }
-        local += cnt[k];
+        const int c = c0 + k;
+        if (c >= HC) break;
+        bool inside;
+        const uint32_t key = halo_key(P, k0, g, c, inside);
+        uint32_t n = inside ? cstart[key + 1] - cstart[key] : 0u;
+        sm.off[c] = (uint16_t)(n < 65535u ? n : 65535u);
+        local += n;
     }
     const int lane = tid & 63, w = tid >> 6;
     uint32_t x = local;
@@ -407,152 +426,201 @@ __device__ bool stage(const DevParams& P, uint32_t k0, const BrickGeo& g, const 
     uint32_t woff = 0, total = 0;
 #pragma unroll
     for (int k = 0; k < BW; ++k) {
-        uint32_t s = sm.wsum[k];
-        if (k < w) woff += s;
-        total += s;
+        uint32_t v = sm.wsum[k];
+        if (k < w) woff += v;
+        total += v;
     }
-    const bool fits = total <= (uint32_t)MAXP;
+    const bool fits = total < 65535u;
     uint32_t run = woff + x - local;
-#pragma unroll
     for (int k = 0; k < PER; ++k) {
-        int c = tid * PER + k;
-        if (c < HC) sm.off[c] = (uint16_t)(fits ? run : 0);
-        if (fits && c < HC) {
-            for (uint32_t m = 0; m < cnt[k]; ++m) {
-                uint32_t jg = start[k] + m;
-                float4 pj = pos[jg];
-                rec(sm, run + m, jg, pj);
-                int ix, iy, iz;
-                cell_trunc(P, pj.x, pj.y, pj.z, ix, iy, iz);
-                if ((unsigned)(ix - g.bx0) < (unsigned)BS && (unsigned)(iy - g.by0) < (unsigned)BS &&
-                    (unsigned)(iz - g.bz0) < (unsigned)BS) {
-                    // owned particles are listed within one cell of the brick (drift < 1 cell);
-                    // counting them only there keeps each one once
-                    int hx = c % H, hy = (c / H) % H, hz = c / (H * H);
-                    if (hx >= S - 1 && hx <= S + BS && hy >= S - 1 && hy <= S + BS && hz >= S - 1 &&
-                        hz <= S + BS) {
-                        int idx = atomicAdd(&sm.nown, 1);
-                        if (idx < MAXOWN) sm.own[idx] = jg;
-                    }
-                }
-            }
-        }
-        run += cnt[k];
+        const int c = c0 + k;
+        if (c >= HC) break;
+        const uint32_t v = sm.off[c];
+        sm.off[c] = (uint16_t)(fits ? run : 0u);
+        run += v;
     }
+    if (tid == 0) sm.off[HC] = (uint16_t)(fits ? total : 0u);
+    __syncthreads();
     if (tid == 0) {
-        sm.off[HC] = (uint16_t)(fits ? total : 0);
-        sm.total = (int)total;
+        const int L2 = g.Hx * g.Hy;
+        int ns = 0, z = 0;
+        bool ok = fits;
+        sm.slab[0] = 0;
+        while (ok && z < g.Hz) {
+            const uint32_t b = sm.off[z * L2];
+            int z1 = z;
+            while (z1 < g.Hz && (uint32_t)sm.off[(z1 + 1) * L2] - b <= (uint32_t)MAXP) ++z1;
+            if (z1 == z) ok = false;
+            else { sm.slab[++ns] = (uint8_t)z1; z = z1; }
+        }
+        sm.nslab = ok ? ns : 0;
     }
     __syncthreads();
-    return fits && sm.nown <= MAXOWN;
 }
 
-// Calls fn(j) for every particle whose current cell is in the brick, found among the particles
-// listed within one cell of it (overflow fallback; thread-strided).
-template <int S, class Fn>
-__device__ void for_each_owned_global(const DevParams& P, uint32_t k0, const BrickGeo& g, const float4* __restrict__ pos,
-                                      const uint32_t* __restrict__ cstart, Fn fn) {
-    constexpr int W = BS + 2;
-    for (int c = threadIdx.x; c < W * W * W; c += BT) {
-        int ux = g.bx0 - 1 + c % W, uy = g.by0 - 1 + (c / W) % W, gz = g.bz0 - 1 + c / (W * W);
-        if (gz < 0 || gz >= P.nz) continue;
-        uint32_t key = k0 | morton3((uint32_t)wrapi(ux, P.nx), (uint32_t)wrapi(uy, P.ny), (uint32_t)gz);
-        for (uint32_t jg = cstart[key]; jg < cstart[key + 1]; ++jg) {
-            float4 pj = pos[jg];
-            int ix, iy, iz;
-            cell_trunc(P, pj.x, pj.y, pj.z, ix, iy, iz);
-            if ((unsigned)(ix - g.bx0) < (unsigned)BS && (unsigned)(iy - g.by0) < (unsigned)BS &&
-                (unsigned)(iz - g.bz0) < (unsigned)BS)
-                fn((int)jg, pj);
-        }
+// Stages slab s (its halo cells' particles, listed order) into LDS. Workgroup-uniform; waits
+// for every wavefront to finish with the previous slab first.
+template <int S, bool MOM, class Rec>
+__device__ __forceinline__ void stage_slab(const DevParams& P, uint32_t k0, const BrickGeo& g, int s,
+                                           const float4* __restrict__ pos, const uint32_t* __restrict__ cstart,
+                                           Smem<S, MOM>& sm, Rec rec) {
+    __syncthreads();
+    const int L2 = g.Hx * g.Hy;
+    const int cb = sm.slab[s] * L2, ce = sm.slab[s + 1] * L2;
+    const uint32_t base = sm.off[cb];
+    for (int c = cb + (int)threadIdx.x; c < ce; c += BT) {
+        const uint32_t o = sm.off[c], n = (uint32_t)sm.off[c + 1] - o;
+        if (n == 0) continue;
+        bool inside;
+        const uint32_t key = halo_key(P, k0, g, c, inside);
+        const uint32_t st = cstart[key];
+        for (uint32_t m = 0; m < n; ++m) rec(sm, o - base + m, st + m, pos[st + m]);
     }
+    __syncthreads();
 }
 
-// Walks the culled stencil rows of an owned particle: calls cand(slot) for every candidate in
-// the reference order.
+// Current cell and cell-unit position of a particle in the brick's frame: a particle listed in
+// the brick may have crossed a periodic face since the sort; its cell and position are shifted
+// by a period so that its stencil rows index the halo (the pair arithmetic keeps the stored
+// coordinates, as the reference does).
+struct Cur {
+    int ix, iy, z0, z1;
+    float xi, yi, zi;
+};
+
+template <int S>
+__device__ __forceinline__ Cur cur_cell(const DevParams& P, const BrickGeo& g, float4 pi) {
+    Cur c;
+    int iz;
+    cell_trunc(P, pi.x, pi.y, pi.z, c.ix, c.iy, iz);
+    zrange<S>(P, iz, c.z0, c.z1);
+    c.xi = pi.x * P.inv_dx;
+    c.yi = pi.y * P.inv_dy;
+    c.zi = pi.z * P.inv_dz;
+    // listed in [b0, b0 + BS) and at most one cell away from it: the offset lies in [-1, BS]
+    const int dx = c.ix - g.bx0, dy = c.iy - g.by0;
+    if (dx > BS) { c.ix -= P.nx; c.xi -= (float)P.nx; }
+    else if (dx < -1) { c.ix += P.nx; c.xi += (float)P.nx; }
+    if (dy > BS) { c.iy -= P.ny; c.yi -= (float)P.ny; }
+    else if (dy < -1) { c.iy += P.ny; c.yi += (float)P.ny; }
+    return c;
+}
+
+// Walks the culled stencil rows of a particle that lie in slab s: calls cand(slot) for every
+// candidate in the reference order. Returns the number of candidates.
 template <int S, class Cand>
-__device__ __forceinline__ void walk_rows(const DevParams& P, const BrickGeo& g, const float* delta,
-                                          const uint16_t* off, float4 pi, Cand cand) {
-    constexpr int H = Halo<S>::H;
-    int ix, iy, iz, z0, z1;
-    cell_trunc(P, pi.x, pi.y, pi.z, ix, iy, iz);
-    zrange<S>(P, iz, z0, z1);
-    const float xi = pi.x * P.inv_dx, yi = pi.y * P.inv_dy, zi = pi.z * P.inv_dz;
+__device__ __forceinline__ int walk_rows(const DevParams& P, const BrickGeo& g, const float* delta,
+                                         const uint16_t* off, const uint8_t* slab, int s, const Cur& c,
+                                         Cand cand) {
     const float limc = P.lim * 1.0001f;
-    const int hx0 = g.bx0 - S, hy0 = g.by0 - S, hz0 = g.bz0 - S;
-    for (int jz = z0; jz <= z1; ++jz) {
-        float ez = fmaxf(0.f, fmaxf(((float)jz - delta[2]) - zi, zi - ((float)(jz + 1) + delta[2])));
+    const int L2 = g.Hx * g.Hy;
+    const int base = off[slab[s] * L2];
+    const int za = max(c.z0, g.hz0 + (int)slab[s]);
+    const int zb = min(c.z1, g.hz0 + (int)slab[s + 1] - 1);
+    const int ya = max(c.iy - S, g.hy0), yb = min(c.iy + S, g.hy0 + g.Hy - 1);
+    int ncand = 0;
+    for (int jz = za; jz <= zb; ++jz) {
+        float ez = fmaxf(0.f, fmaxf(((float)jz - delta[2]) - c.zi, c.zi - ((float)(jz + 1) + delta[2])));
         float ez2 = ez * ez;
-        const int rowz = (jz - hz0) * H;
-        for (int jy = iy - S; jy <= iy + S; ++jy) {
-            float ey = fmaxf(0.f, fmaxf(((float)jy - delta[1]) - yi, yi - ((float)(jy + 1) + delta[1])));
+        const int rowz = (jz - g.hz0) * g.Hy;
+        for (int jy = ya; jy <= yb; ++jy) {
+            float ey = fmaxf(0.f, fmaxf(((float)jy - delta[1]) - c.yi, c.yi - ((float)(jy + 1) + delta[1])));
             float d2 = __builtin_fmaf(ey, ey, ez2);
             if (d2 >= limc) continue;
             float ext = sqrtf(limc - d2) + DELTA_EPS;
-            int xlo = max(ix - S, (int)ceilf(xi - ext - 1.0f - delta[0]));
-            int xhi = min(ix + S, (int)floorf(xi + ext + delta[0]));
+            int xlo = max(max(c.ix - S, g.hx0), (int)ceilf(c.xi - ext - 1.0f - delta[0]));
+            int xhi = min(min(c.ix + S, g.hx0 + g.Hx - 1), (int)floorf(c.xi + ext + delta[0]));
             if (xlo > xhi) continue;
-            const int row = (rowz + (jy - hy0)) * H - hx0;
-            const int a = off[row + xlo], e = off[row + xhi + 1];
+            const int row = (rowz + (jy - g.hy0)) * g.Hx - g.hx0;
+            const int a = off[row + xlo] - base, e = off[row + xhi + 1] - base;
+            ncand += e - a;
             for (int p = a; p < e; ++p) cand(p);
         }
     }
+    return ncand;
+}
+
+// Adds a per-lane count, summed over the wavefront, to a work counter.
+__device__ __forceinline__ void count_wave(unsigned long long* stats, int idx, int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&stats[idx * 64 + (blockIdx.x & 63)], (unsigned long long)v);
+}
+
+__device__ __forceinline__ void count_wg(unsigned long long* stats, int idx, int v = 1) {
+    if (stats && threadIdx.x == 0) atomicAdd(&stats[idx * 64 + (blockIdx.x & 63)], (unsigned long long)v);
 }
 
 // ---------------------------------------------------------------------------------------------
 // Density + EOS + microphysics.
 // ---------------------------------------------------------------------------------------------
-template <int S, bool WRAP>
-__device__ void dens_own(const DevParams& P, const DensArgs& A, const DensConst& K, const BrickGeo& g,
-                         Smem<S, false>& sm, int i, bool active) {
+template <int S, bool WRAP, class Rec>
+__device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A, const DensConst& K,
+                                           uint32_t k0, const BrickGeo& g, uint32_t o0, uint32_t o1,
+                                           Smem<S, false>& sm, Rec rec) {
     const int tid = threadIdx.x;
-    float4 pi = active ? A.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
-    float S1 = 0.f, S2 = 0.f;
-    int qn = 0;
-    auto diff = [&](float4 pj, float& dxm, float& dym, float& dzm) {
-        float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-        if (WRAP) min_image(P, rx, ry);
-        dxm = rx * P.inv_dx; dym = ry * P.inv_dy; dzm = rz * P.inv_dz;
-    };
-    auto flush = [&]() {
-        for (int k = 0; wave_any(k < qn); ++k) {
-            if (k < qn) {
-                float4 pj = sm.p[sm.q[k][tid]];
-                float dxm, dym, dzm;
-                diff(pj, dxm, dym, dzm);
-                float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
-                float W = fker(sqrtf(r2) * P.inv_h);
-                S1 += W;
-                S2 += W * pj.w;
+    const int ns = sm.nslab;
+    if (ns == 1) stage_slab<S, false>(P, k0, g, 0, A.pos, A.cstart, sm, rec);
+    for (uint32_t r = o0; r < o1; r += BT) {
+        const uint32_t i = r + tid;
+        const bool active = i < o1;
+        const float4 pi = active ? A.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
+        const Cur cc = cur_cell<S>(P, g, pi);
+        float S1 = 0.f, S2 = 0.f;
+        int qn = 0, npair = 0, ncand = 0;
+        auto diff = [&](float4 pj, float& dxm, float& dym, float& dzm) {
+            float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
+            if (WRAP) min_image(P, rx, ry);
+            dxm = rx * P.inv_dx; dym = ry * P.inv_dy; dzm = rz * P.inv_dz;
+        };
+        auto flush = [&]() {
+            for (int k = 0; wave_any(k < qn); ++k) {
+                if (k < qn) {
+                    float4 pj = sm.p[sm.q[k][tid]];
+                    float dxm, dym, dzm;
+                    diff(pj, dxm, dym, dzm);
+                    float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+                    float W = fker(sqrtf(r2) * P.inv_h);
+                    S1 += W;
+                    S2 += W * pj.w;
+                }
             }
+            npair += qn;
+            qn = 0;
+        };
+        for (int s = 0; s < ns; ++s) {
+            if (ns > 1) stage_slab<S, false>(P, k0, g, s, A.pos, A.cstart, sm, rec);
+            if (active) {
+                ncand += walk_rows<S>(P, g, sm.delta, sm.off, sm.slab, s, cc, [&](int p) {
+                    float4 pj = sm.p[p];
+                    float dxm, dym, dzm;
+                    diff(pj, dxm, dym, dzm);
+                    float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+                    if (r2 < P.lim) {
+                        sm.q[qn][tid] = (uint16_t)p;
+                        ++qn;
+                    }
+                    if (wave_any(qn == QK)) flush();
+                });
+            }
+            flush();
         }
-        qn = 0;
-    };
-    if (active) {
-        walk_rows<S>(P, g, sm.delta, sm.off, pi, [&](int p) {
-            float4 pj = sm.p[p];
-            float dxm, dym, dzm;
-            diff(pj, dxm, dym, dzm);
-            float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
-            if (r2 < P.lim) {
-                sm.q[qn][tid] = (uint16_t)p;
-                ++qn;
-            }
-            if (wave_any(qn == QK)) flush();
-        });
+        if (active) dens_epilogue(P, A, K, (int)i, pi, S1, S2);
+        if (A.stats) {
+            count_wave(A.stats, ASPH_C_DENS_CAND, ncand);
+            count_wave(A.stats, ASPH_C_DENS_PAIRS, npair);
+        }
     }
-    flush();
-    if (active) dens_epilogue(P, A, K, i, pi, S1, S2);
 }
 
 template <int S>
-__global__ __launch_bounds__(BT) void k_density_eos_brick(const MemberDesc* __restrict__ mem,
-                                                          const BlockMap* __restrict__ bmap, DensArgs A) {
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_density_eos_brick(
+        const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, DensArgs A) {
     __shared__ Smem<S, false> sm;
     const int tid = threadIdx.x;
     const BlockMap bm = bmap[blockIdx.x];
     const MemberDesc& MD = mem[bm.m];
-    const DevParams P = MD.P;
+    const DevParams& P = MD.P;
     const uint32_t k0 = MD.k0;
     if (bm.ord == 0 && tid < 3 * 64) A.maxv[bm.m * 192 + tid] = 0u;   // this member's max|v| slots
     DensConst K;
@@ -561,6 +629,7 @@ __global__ __launch_bounds__(BT) void k_density_eos_brick(const MemberDesc* __re
     sfc_params(P, A.clock ? A.clock[0] : A.simt, K.Ts, K.qs_sfc);
     prologue<S, false>(P, A.disp, A.disp_slots, sm);
     if (!sm.modeA) {  // mode B: particle per thread by index within the member
+        count_wg(A.stats, ASPH_C_DENS_MODEB);
         int il = bm.ord * BT + tid;
         if (il < P.N) {
             int i = MD.p0 + il;
@@ -571,83 +640,102 @@ __global__ __launch_bounds__(BT) void k_density_eos_brick(const MemberDesc* __re
         }
         return;
     }
-    const BrickGeo g = brick_geo(P, S, (uint32_t)bm.base);
-    const float* rho_old = A.rho_old;
-    bool ok = stage<S, false>(P, k0, g, A.pos, A.cstart, sm, [&](Smem<S, false>& s, uint32_t slot, uint32_t jg, float4 pj) {
-        s.p[slot] = make_float4(pj.x, pj.y, pj.z, 1.0f / fmaxf(rho_old[jg], 1e-3f));
-    });
-    if (!ok) {
-        for_each_owned_global<S>(P, k0, g, A.pos, A.cstart, [&](int i, float4 pi) {
+    // owned: the particles listed in the brick's 512 cells, one contiguous range
+    const uint32_t kb = (uint32_t)bm.base;
+    const uint32_t o0 = A.cstart[k0 + kb], o1 = A.cstart[k0 + kb + (1u << NB)];
+    if (o0 == o1) return;
+    const BrickGeo g = brick_geo<S, false>(P, sm, kb);
+    plan<S, false>(P, k0, g, A.cstart, sm);
+    if (sm.nslab == 0) {
+        count_wg(A.stats, ASPH_C_DENS_GLOBAL);
+        for (uint32_t i = o0 + tid; i < o1; i += BT) {
+            float4 pi = A.pos[i];
             float S1, S2;
             dens_sum_global<S>(P, k0, pi, A.pos, A.rho_old, A.cstart, S1, S2);
-            dens_epilogue(P, A, K, i, pi, S1, S2);
-        });
+            dens_epilogue(P, A, K, (int)i, pi, S1, S2);
+        }
         return;
     }
-    const int nown = sm.nown;
-    for (int r = 0; r < nown; r += BT) {
-        int k = r + tid;
-        bool active = k < nown;
-        int i = active ? (int)sm.own[k] : 0;
-        if (g.wraps) dens_own<S, true>(P, A, K, g, sm, i, active);
-        else dens_own<S, false>(P, A, K, g, sm, i, active);
-    }
+    count_wg(A.stats, sm.nslab == 1 ? ASPH_C_DENS_ONE_SLAB : ASPH_C_DENS_SLABBED);
+    count_wg(A.stats, ASPH_C_DENS_STAGINGS, sm.nslab * (int)((o1 - o0 + BT - 1) / BT));
+    const float* rho_old = A.rho_old;
+    auto rec = [&](Smem<S, false>& s, uint32_t slot, uint32_t jg, float4 pj) {
+        s.p[slot] = make_float4(pj.x, pj.y, pj.z, 1.0f / fmaxf(rho_old[jg], 1e-3f));
+    };
+    if (g.wraps) dens_brick<S, true>(P, A, K, k0, g, o0, o1, sm, rec);
+    else dens_brick<S, false>(P, A, K, k0, g, o0, o1, sm, rec);
 }
 
 // ---------------------------------------------------------------------------------------------
 // Momentum + theta update + kick + sponge + max|v|.
 // ---------------------------------------------------------------------------------------------
-template <int S, bool WRAP>
-__device__ void mom_own(const DevParams& P, const MomArgs& A, const BrickGeo& g, Smem<S, true>& sm,
-                        int i, bool active, float& mx, float& my, float& mz) {
+template <int S, bool WRAP, class Rec>
+__device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, uint32_t k0, const BrickGeo& g,
+                                          uint32_t o0, uint32_t o1, Smem<S, true>& sm, Rec rec,
+                                          float& mx, float& my, float& mz) {
     const int tid = threadIdx.x;
-    float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi, ti = pi;
-    float rhoi = 1.f, pri = 0.f;
-    if (active) {
-        pi = A.pos[i];
-        vi = A.vel[i];
-        ti = A.thq[i];
-        rhoi = fmaxf(A.rho[i], 1e-3f);
-        pri = A.prs[i];
-    }
-    const float inv_rhoi = 1.0f / rhoi;
-    MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-    int qn = 0;
-    auto diff = [&](float4 pj, float& rx, float& ry, float& rz) {
-        rx = pi.x - pj.x; ry = pi.y - pj.y; rz = pi.z - pj.z;
-        if (WRAP) min_image(P, rx, ry);
-    };
-    auto flush = [&]() {
-        for (int k = 0; wave_any(k < qn); ++k) {
-            if (k < qn) {
-                int p = sm.q[k][tid];
-                float4 pj = sm.p[p];
-                float4 vj = A.vel[sm.j[p]];   // (vx, vy, vz, p'_j)
-                float rx, ry, rz;
-                diff(pj, rx, ry, rz);
-                float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
-                float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
-                mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, pj.w, vj, vj.w, M);
-            }
+    const int ns = sm.nslab;
+    if (ns == 1) stage_slab<S, true>(P, k0, g, 0, A.pos, A.cstart, sm, rec);
+    for (uint32_t r = o0; r < o1; r += BT) {
+        const uint32_t i = r + tid;
+        const bool active = i < o1;
+        float4 pi = make_float4(0.f, 0.f, 0.f, 0.f), vi = pi, ti = pi;
+        float rhoi = 1.f, pri = 0.f;
+        if (active) {
+            pi = A.pos[i];
+            vi = A.vel[i];
+            ti = A.thq[i];
+            rhoi = fmaxf(A.rho[i], 1e-3f);
+            pri = A.prs[i];
         }
-        qn = 0;
-    };
-    if (active) {
-        walk_rows<S>(P, g, sm.delta, sm.off, pi, [&](int p) {
-            float4 pj = sm.p[p];
-            float rx, ry, rz;
-            diff(pj, rx, ry, rz);
-            float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
-            float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
-            if (r2t < P.lim && r2t >= 1e-12f) {
-                sm.q[qn][tid] = (uint16_t)p;
-                ++qn;
+        const Cur cc = cur_cell<S>(P, g, pi);
+        const float inv_rhoi = 1.0f / rhoi;
+        MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+        int qn = 0, npair = 0, ncand = 0;
+        auto diff = [&](float4 pj, float& rx, float& ry, float& rz) {
+            rx = pi.x - pj.x; ry = pi.y - pj.y; rz = pi.z - pj.z;
+            if (WRAP) min_image(P, rx, ry);
+        };
+        auto flush = [&]() {
+            for (int k = 0; wave_any(k < qn); ++k) {
+                if (k < qn) {
+                    int p = sm.q[k][tid];
+                    float4 pj = sm.p[p];
+                    float4 vj = A.vel[sm.j[p]];   // (vx, vy, vz, p'_j)
+                    float rx, ry, rz;
+                    diff(pj, rx, ry, rz);
+                    float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
+                    float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
+                    mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, pj.w, vj, vj.w, M);
+                }
             }
-            if (wave_any(qn == QK)) flush();
-        });
+            npair += qn;
+            qn = 0;
+        };
+        for (int s = 0; s < ns; ++s) {
+            if (ns > 1) stage_slab<S, true>(P, k0, g, s, A.pos, A.cstart, sm, rec);
+            if (active) {
+                ncand += walk_rows<S>(P, g, sm.delta, sm.off, sm.slab, s, cc, [&](int p) {
+                    float4 pj = sm.p[p];
+                    float rx, ry, rz;
+                    diff(pj, rx, ry, rz);
+                    float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
+                    float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
+                    if (r2t < P.lim && r2t >= 1e-12f) {
+                        sm.q[qn][tid] = (uint16_t)p;
+                        ++qn;
+                    }
+                    if (wave_any(qn == QK)) flush();
+                });
+            }
+            flush();
+        }
+        if (active) mom_epilogue(P, A, (int)i, pi, vi, ti, pri, M, mx, my, mz);
+        if (A.stats) {
+            count_wave(A.stats, ASPH_C_MOM_CAND, ncand);
+            count_wave(A.stats, ASPH_C_MOM_PAIRS, npair);
+        }
     }
-    flush();
-    if (active) mom_epilogue(P, A, i, pi, vi, ti, pri, M, mx, my, mz);
 }
 
 __device__ __forceinline__ void reduce_maxv(uint32_t* maxv, float mx, float my, float mz) {
@@ -666,18 +754,19 @@ __device__ __forceinline__ void reduce_maxv(uint32_t* maxv, float mx, float my, 
 }
 
 template <int S>
-__global__ __launch_bounds__(BT) void k_momentum_kdk_brick(const MemberDesc* __restrict__ mem,
-                                                           const BlockMap* __restrict__ bmap, MomArgs A) {
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_momentum_kdk_brick(
+        const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, MomArgs A) {
     __shared__ Smem<S, true> sm;
     const int tid = threadIdx.x;
     const BlockMap bm = bmap[blockIdx.x];
     const MemberDesc& MD = mem[bm.m];
-    const DevParams P = MD.P;
+    const DevParams& P = MD.P;
     const uint32_t k0 = MD.k0;
     uint32_t* maxv = A.maxv + bm.m * 192;
     float mx = 0.f, my = 0.f, mz = 0.f;
     prologue<S, true>(P, A.disp, A.disp_slots, sm);
     if (!sm.modeA) {
+        count_wg(A.stats, ASPH_C_MOM_MODEB);
         int il = bm.ord * BT + tid;
         if (il < P.N) {
             int i = MD.p0 + il;
@@ -690,29 +779,30 @@ __global__ __launch_bounds__(BT) void k_momentum_kdk_brick(const MemberDesc* __r
         if (A.mode) reduce_maxv(maxv, mx, my, mz);
         return;
     }
-    const BrickGeo g = brick_geo(P, S, (uint32_t)bm.base);
-    const float* rho = A.rho;
-    bool ok = stage<S, true>(P, k0, g, A.pos, A.cstart, sm, [&](Smem<S, true>& s, uint32_t slot, uint32_t jg, float4 pj) {
-        s.p[slot] = make_float4(pj.x, pj.y, pj.z, fmaxf(rho[jg], 1e-3f));
-        s.j[slot] = jg;
-    });
-    if (!ok) {
-        for_each_owned_global<S>(P, k0, g, A.pos, A.cstart, [&](int i, float4 pi) {
-            float4 vi = A.vel[i], ti = A.thq[i];
+    const uint32_t kb = (uint32_t)bm.base;
+    const uint32_t o0 = A.cstart[k0 + kb], o1 = A.cstart[k0 + kb + (1u << NB)];
+    if (o0 == o1) return;
+    const BrickGeo g = brick_geo<S, true>(P, sm, kb);
+    plan<S, true>(P, k0, g, A.cstart, sm);
+    if (sm.nslab == 0) {
+        count_wg(A.stats, ASPH_C_MOM_GLOBAL);
+        for (uint32_t i = o0 + tid; i < o1; i += BT) {
+            float4 pi = A.pos[i], vi = A.vel[i], ti = A.thq[i];
             float rhoi = fmaxf(A.rho[i], 1e-3f), pri = A.prs[i];
             MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
             mom_sum_global<S>(P, k0, pi, vi, rhoi, pri, A.pos, A.vel, A.rho, A.prs, A.cstart, M);
-            mom_epilogue(P, A, i, pi, vi, ti, pri, M, mx, my, mz);
-        });
-    } else {
-        const int nown = sm.nown;
-        for (int r = 0; r < nown; r += BT) {
-            int k = r + tid;
-            bool active = k < nown;
-            int i = active ? (int)sm.own[k] : 0;
-            if (g.wraps) mom_own<S, true>(P, A, g, sm, i, active, mx, my, mz);
-            else mom_own<S, false>(P, A, g, sm, i, active, mx, my, mz);
+            mom_epilogue(P, A, (int)i, pi, vi, ti, pri, M, mx, my, mz);
         }
+    } else {
+        count_wg(A.stats, sm.nslab == 1 ? ASPH_C_MOM_ONE_SLAB : ASPH_C_MOM_SLABBED);
+        count_wg(A.stats, ASPH_C_MOM_STAGINGS, sm.nslab * (int)((o1 - o0 + BT - 1) / BT));
+        const float* rho = A.rho;
+        auto rec = [&](Smem<S, true>& s, uint32_t slot, uint32_t jg, float4 pj) {
+            s.p[slot] = make_float4(pj.x, pj.y, pj.z, fmaxf(rho[jg], 1e-3f));
+            s.j[slot] = jg;
+        };
+        if (g.wraps) mom_brick<S, true>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
+        else mom_brick<S, false>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
     }
     if (A.mode) reduce_maxv(maxv, mx, my, mz);
 }

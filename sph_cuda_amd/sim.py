@@ -260,5 +260,15 @@ class Sim:
                 "asph_kernel_times")
         return {L.KERNEL_NAMES[k]: (float(ms[k]), int(cnt[k])) for k in range(L.K_COUNT)}
 
+    def set_counters(self, on: bool):
+        L.check(self.lib.asph_set_counters(self.h, 1 if on else 0), "asph_set_counters")
+
+    def counters(self) -> dict:
+        """Work counters of the neighbour kernels since the last call (see include/asph.h)."""
+        out = np.zeros(L.C_COUNT, np.uint64)
+        L.check(self.lib.asph_get_counters(self.h, out.ctypes.data_as(C.POINTER(C.c_uint64))),
+                "asph_get_counters")
+        return {n: int(out[k]) for k, n in enumerate(L.COUNTER_NAMES)}
+
     def synchronize(self):
         L.check(self.lib.asph_synchronize(self.h), "asph_synchronize")

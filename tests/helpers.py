@@ -60,3 +60,4 @@ def state_close(a: s.State, b: s.State, tol: float):
             out[name] = relnorm(x, y)
     bad = {k: v for k, v in out.items() if v > tol}
     return bad, out
+

@@ -216,6 +216,47 @@ def test_clumped_overflow_fallback():
         assert relnorm(ga[:, c], oa[:, c]) <= TOL
 
 
+def _compressed(a):
+    """The moist configuration on 32 x 32 x 24 cells with every particle position scaled by 1/a
+    towards the origin: number density a^3 in a region wider than a brick halo."""
+    cfg, ic, dt = small_config("moist", (32, 32, 24))
+    st = s.ic_generate(cfg, ic)
+    st.pos[:, :3] = (st.pos[:, :3] / np.float32(a)).astype(np.float32)
+    return cfg, dt, st
+
+
+# (compression, the path the densest bricks must take): a halo beyond the LDS capacity is
+# streamed through it in z-slabs; a brick with one halo layer beyond it takes the global loop
+SPLITS = [(1.2, "slabbed"), (1.7, "slabbed"), (3.5, "global")]
+
+
+@pytest.mark.parametrize("a,path", SPLITS)
+def test_dense_bricks(a, path):
+    cfg, dt, st = _compressed(a)
+    print(f"[case] compression {a}", flush=True)
+    g, o = s.Sim(cfg), Oracle(cfg)
+    g.set_counters(True)
+    g.set_state(st)
+    o.set_state(st)
+    for x in (g, o):
+        x.sort()
+        x.force(dt)
+    c = g.counters()
+    print(f"[case] counters {c}", flush=True)
+    assert c["dens_" + path] > 0 and c["mom_" + path] > 0
+    if path == "slabbed":
+        assert c["dens_global"] == 0 and c["dens_stagings"] > c["dens_slabbed"]
+    ga, gp, _ = g.get_aux()
+    oa, op, _ = o.get_aux()
+    assert np.array_equal(g.get_state().rho, o.get_state().rho)
+    assert np.array_equal(ga, oa) and np.array_equal(gp, op)
+    for k in range(3):
+        for x in (g, o):
+            x.step(dt, k * dt, do_sort=True)
+        bad, errs = state_close(g.get_state(), o.get_state(), TOL)
+        assert not bad, (k, errs)
+
+
 def test_large_drift_without_sort():
     """Steps without re-sorting while particles drift more than one cell: the kernels switch to
     the thread-per-particle path (drift bound >= 0.95 cell) and still match the oracle, which
