@@ -32,6 +32,11 @@ B_STAGE = dict(kick_drift=48 + 48, density_eos=32 + 56, momentum_kdk=84 + 36 + 4
                rain=8.0)
 B_STEP = 384.8
 B_SORT = 212.0
+# HBM bytes per particle per launch measured with FETCH_SIZE / WRITE_SIZE counters on this bench
+# command (scaled by a 1 GiB device copy read back through the same counters), averaged over the
+# 300 timed steps of the 256^3 workload; profiles/step_256cubed_hbm_traffic.md holds the table and
+# the commands. Reported as roofline.traffic for the 256^3 workload only.
+TRAFFIC_B_PER_PARTICLE = dict(momentum_kdk=350.6, density_eos=285.5, kick_drift=88.5, rain=24.6)
 
 
 def log(msg):
@@ -134,6 +139,9 @@ def main():
     avg_s = ms_total / 1e3 / max(cnt, 1)
     bytes_launch = B_STAGE[dom] * n
     achieved = bytes_launch / avg_s / 1e9
+    traffic = None
+    if args.preset == 3 and not args.n and args.steps == 300 and args.warmup == 10:
+        traffic = TRAFFIC_B_PER_PARTICLE[dom] * n
     step_alg_gbs = (B_STEP * n * args.steps + B_SORT * n * ts.sorts) / wall / 1e9
 
     out = {
@@ -162,7 +170,8 @@ def main():
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
             "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "alg_bytes_per_particle": B_STAGE[dom], "traffic": None,
+            "alg_bytes_per_particle": B_STAGE[dom], "traffic": traffic,
+            "traffic_GBps_at_measured_time": (traffic / avg_s / 1e9) if traffic else None,
         },
         "cpu_baseline": None,
     }

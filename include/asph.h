@@ -305,6 +305,9 @@ int asph_synchronize(asph_t* h);
 #define ASPH_C_MOM_MODEB     11
 #define ASPH_C_MOM_CAND      12
 #define ASPH_C_MOM_PAIRS     13
+#define ASPH_C_DENS_WAVE_CAND 14 /* density: sum over wavefronts of 64 x the largest per-lane   */
+                                 /*          candidate count (lane-slots spent on the walk)     */
+#define ASPH_C_MOM_WAVE_CAND 15  /* momentum: as 14                                           */
 #define ASPH_C_COUNT         16
 /* Enables (allocating and zeroing the counters) or disables counting. */
 int asph_set_counters(asph_t* h, int32_t enable);

@@ -28,7 +28,8 @@ K_KICK_DRIFT, K_DENSITY_EOS, K_MOMENTUM_KDK, K_RAIN, K_SORT = range(5)
 C_COUNT = 16
 COUNTER_NAMES = ("dens_one_slab", "dens_slabbed", "dens_stagings", "dens_global", "dens_modeb",
                  "dens_cand", "dens_pairs", "mom_one_slab", "mom_slabbed", "mom_stagings",
-                 "mom_global", "mom_modeb", "mom_cand", "mom_pairs")
+                 "mom_global", "mom_modeb", "mom_cand", "mom_pairs", "dens_wave_cand",
+                 "mom_wave_cand")
 K_COUNT = 5
 KERNEL_NAMES = ("kick_drift", "density_eos", "momentum_kdk", "rain", "sort")
 

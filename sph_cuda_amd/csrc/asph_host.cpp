@@ -187,13 +187,25 @@ int create_impl(const asph_config* cfgs, int n, void* stream, asph_t** out) {
     rc |= h->alloc(&h->mem_d, h->mem_h.size());
     rc |= h->alloc(&h->pmap_d, h->pmap_h.size());
     rc |= h->alloc(&h->bmap_d, h->bmap_h.size());
+    const int nitem_max = brick_item_capacity((int)h->bmap_h.size(), h->N);
+    int2* items = nullptr;
+    uint32_t *nitems = nullptr, *item_hist = nullptr, *item_meta = nullptr;
+    rc |= h->alloc(&items, (size_t)nitem_max);
+    rc |= h->alloc(&nitems, 1);
+    rc |= h->alloc(&item_hist, (size_t)2 * brick_item_bins());
+    rc |= h->alloc(&item_meta, h->bmap_h.size());
     if (rc != ASPH_OK) { std::string e = g_err; asph_destroy(h); g_err = e; return ASPH_ERR; }
     hipError_t e = hipMemcpy(h->mem_d, h->mem_h.data(), h->mem_h.size() * sizeof(MemberDesc), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->pmap_d, h->pmap_h.data(), h->pmap_h.size() * sizeof(BlockMap), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->bmap_d, h->bmap_h.data(), h->bmap_h.size() * sizeof(BlockMap), hipMemcpyHostToDevice);
+    // the item histogram must start at zero (each build clears it again); no items before a sort
+    if (e == hipSuccess) e = hipMemset(item_hist, 0, (size_t)2 * brick_item_bins() * sizeof(uint32_t));
+    if (e == hipSuccess) e = hipMemset(nitems, 0, sizeof(uint32_t));
     if (e != hipSuccess) { asph_destroy(h); return fail("table upload: %s", hipGetErrorString(e)); }
     h->L.mem = h->mem_d; h->L.pmap = h->pmap_d; h->L.bmap = h->bmap_d;
     h->L.npblk = (int)h->pmap_h.size(); h->L.nbblk = (int)h->bmap_h.size(); h->L.n_members = n;
+    h->L.items = items; h->L.nitems = nitems; h->L.item_hist = item_hist; h->L.item_meta = item_meta;
+    h->L.nitem_max = nitem_max;
     *out = h;
     return ASPH_OK;
 }
@@ -219,6 +231,7 @@ void do_sort(asph_t* h, bool launch) {
         launch_gather(h->stream, (int)h->N, h->perm, h->pos, h->pos2, h->vel, h->vel2, h->thq,
                       h->thq2, h->rho, h->rho2, ga ? h->acc : nullptr, ga ? h->acc2 : nullptr);
         launch_cell_start(h->stream, (int)h->N, h->nkeys, h->keys_sorted, h->cstart);
+        launch_brick_items(h->L, h->stream, h->cstart);
         launch_step_begin(h->stream, h->disp, h->disp_slots, 1, h->clock, 2, 0.0, 0.f);
     } else {
         // the sorted keys and permutation land in A or B depending on the pass count

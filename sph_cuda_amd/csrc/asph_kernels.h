@@ -14,6 +14,12 @@ struct Launch {
     int npblk;
     int nbblk;
     int n_members;
+    // work items of the brick kernels, rebuilt after every sort (launch_brick_items)
+    int2* items;             // [nitem_max]: (brick, chunk), heaviest first
+    uint32_t* nitems;        // number of valid items
+    uint32_t* item_hist;     // [2 * brick_item_bins()] counting-sort scratch (zero between builds)
+    uint32_t* item_meta;     // [nbblk] scratch
+    int nitem_max;           // grid of the brick kernels
 };
 
 struct DensArgs {
@@ -63,6 +69,9 @@ void launch_kick_drift(const Launch& L, hipStream_t s, float4* pos, float4* vel,
 void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A);
 void launch_momentum_kdk(const Launch& L, int S, hipStream_t s, const MomArgs& A);
 int brick_bits();
+int brick_item_bins();
+int brick_item_capacity(int nbblk, int64_t n);
+void launch_brick_items(const Launch& L, hipStream_t s, const uint32_t* cstart);
 void launch_rain(const Launch& L, hipStream_t s, const int64_t* rain_in, const int64_t* dep,
                  int64_t* rain_out, int64_t* precip, unsigned long long* clipped, float dt);
 void launch_rain_merge(hipStream_t s, int n, int64_t* rain, int64_t* dep);

@@ -44,8 +44,8 @@ constexpr int NB = 9;              // Morton bits per brick: 8 x 8 x 8 cells
 constexpr int BS = 8;              // brick edge in cells
 constexpr int BT = 512;            // threads per workgroup (one per brick cell)
 constexpr int BW = BT / 64;        // wavefronts per workgroup
-constexpr int MAXP = 3072;         // halo particles held in LDS at a time
 constexpr int QK = 8;              // queued in-support pairs per thread
+constexpr int CHUNK = 768;         // listed particles per work item (at most)
 constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cells]
 
 __device__ __forceinline__ void min_image(const DevParams& P, float& rx, float& ry) {
@@ -312,19 +312,22 @@ __device__ __forceinline__ void mom_epilogue(const DevParams& P, const MomArgs& 
 template <int S> struct HaloMax {
     static constexpr int H = BS + 2 * (S + 1);
     static constexpr int HC = H * H * H;
+    // halo particles held in LDS at a time: what 80 KiB (two workgroups per CU) leaves beside
+    // the cell table and the queues, at most 4032
+    static constexpr int MAXP_FIT = ((81920 - (2 * (HC + 1) + 2 * QK * BT + 256)) / 16) / 64 * 64;
+    static constexpr int MAXP = MAXP_FIT < 4032 ? MAXP_FIT : 4032;
 };
 
 template <int S, bool MOM> struct Smem {
-    float4 p[MAXP];                    // x, y, z, and 1/max(rho_old,1e-3) or max(rho,1e-3)
-    uint32_t j[MOM ? MAXP : 1];        // global index of each staged particle (momentum)
+    float4 p[HaloMax<S>::MAXP];        // x, y, z, and 1/max(rho_old,1e-3) (density) or the
+                                       // particle's global index (momentum)
     uint16_t off[HaloMax<S>::HC + 1];  // exclusive start of each halo cell over the whole halo
     uint16_t q[QK][BT];                // per-thread queue of LDS slots
     uint32_t wsum[BW];
     uint8_t slab[HaloMax<S>::H + 1];   // slab boundaries in halo z-layers
     int nslab;                         // 0: the halo cannot be staged (global-memory loop)
-    int modeA;
-    int mrg[3];                        // halo margin per axis [cells]
-    float delta[3];
+    int modeA;                         // drift bound below 0.95 cell on every axis
+    float delta[3];                    // drift bound since the last sort + rounding margin [cells]
 };
 
 struct BrickGeo {
@@ -334,13 +337,9 @@ struct BrickGeo {
     bool wraps;         // the halo crosses a periodic face
 };
 
-// Reads the drift bound since the last sort and decides between mode A and mode B; in mode A
-// also the halo margins. A particle listed in cell c lies within [c - d, c + 1 + d] (d the drift
-// bound), its current cell within c +- 1, and the cells the culled walk visits within
-// c +- min(S + 1, floor(1 + 2d + R)), R the culling radius: the margin per axis.
+// Reads the drift bound since the last sort (all members) and whether it allows mode A.
 template <int S, bool MOM>
-__device__ __forceinline__ void prologue(const DevParams& P, const float* disp, const uint32_t* disp_slots,
-                                         Smem<S, MOM>& sm) {
+__device__ __forceinline__ void prologue(const float* disp, const uint32_t* disp_slots, Smem<S, MOM>& sm) {
     const int tid = threadIdx.x;
     if (tid < 64) {
         float m[3];
@@ -352,21 +351,27 @@ __device__ __forceinline__ void prologue(const DevParams& P, const float* disp, 
             m[a] = v;
         }
         if (tid == 0) {
-            const float R = sqrtf(P.lim * 1.0001f) + DELTA_EPS;
             bool ok = true;
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 float d = disp[a] + m[a];
                 sm.delta[a] = d + DELTA_EPS;
                 ok = ok && (d < 0.95f);
-                int mg = (int)floorf(1.0f + 2.0f * sm.delta[a] + R);
-                sm.mrg[a] = mg < S + 1 ? mg : S + 1;
             }
-            ok = ok && P.nx >= BS + 2 * sm.mrg[0] && P.ny >= BS + 2 * sm.mrg[1];
             sm.modeA = ok ? 1 : 0;
         }
     }
     __syncthreads();
+}
+
+// Halo margin on one axis for a member: a particle lies within [c - d, c + 1 + d] of its listed
+// cell c (d the drift bound), its current cell within c +- 1, and the cells its culled walk
+// visits within c +- min(S + 1, floor(1 + 2d + R)), R the culling radius.
+template <int S>
+__device__ __forceinline__ int margin_of(const DevParams& P, float d) {
+    const float R = sqrtf(P.lim * 1.0001f) + DELTA_EPS;
+    const int mg = (int)floorf(1.0f + 2.0f * d + R);
+    return mg < S + 1 ? mg : S + 1;
 }
 
 template <int S, bool MOM>
@@ -375,12 +380,14 @@ __device__ __forceinline__ BrickGeo brick_geo(const DevParams& P, const Smem<S, 
     g.bx0 = (int)compact1by2(kb);
     g.by0 = (int)compact1by2(kb >> 1);
     g.bz0 = (int)compact1by2(kb >> 2);
-    g.Hx = BS + 2 * sm.mrg[0];
-    g.Hy = BS + 2 * sm.mrg[1];
-    g.Hz = BS + 2 * sm.mrg[2];
-    g.hx0 = g.bx0 - sm.mrg[0];
-    g.hy0 = g.by0 - sm.mrg[1];
-    g.hz0 = g.bz0 - sm.mrg[2];
+    const int mx = margin_of<S>(P, sm.delta[0]), my = margin_of<S>(P, sm.delta[1]),
+              mz = margin_of<S>(P, sm.delta[2]);
+    g.Hx = BS + 2 * mx;
+    g.Hy = BS + 2 * my;
+    g.Hz = BS + 2 * mz;
+    g.hx0 = g.bx0 - mx;
+    g.hy0 = g.by0 - my;
+    g.hz0 = g.bz0 - mz;
     g.wraps = (g.hx0 < 0) || (g.hx0 + g.Hx > P.nx) || (g.hy0 < 0) || (g.hy0 + g.Hy > P.ny);
     return g;
 }
@@ -449,7 +456,7 @@ __device__ __forceinline__ void plan(const DevParams& P, uint32_t k0, const Bric
         while (ok && z < g.Hz) {
             const uint32_t b = sm.off[z * L2];
             int z1 = z;
-            while (z1 < g.Hz && (uint32_t)sm.off[(z1 + 1) * L2] - b <= (uint32_t)MAXP) ++z1;
+            while (z1 < g.Hz && (uint32_t)sm.off[(z1 + 1) * L2] - b <= (uint32_t)HaloMax<S>::MAXP) ++z1;
             if (z1 == z) ok = false;
             else { sm.slab[++ns] = (uint8_t)z1; z = z1; }
         }
@@ -540,6 +547,14 @@ __device__ __forceinline__ int walk_rows(const DevParams& P, const BrickGeo& g, 
     return ncand;
 }
 
+// Adds 64 x the largest per-lane count of the wavefront (the lane-slots the wavefront spends) to
+// a work counter.
+__device__ __forceinline__ void count_wave_max(unsigned long long* stats, int idx, int v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0 && v) atomicAdd(&stats[idx * 64 + (blockIdx.x & 63)], 64ull * (unsigned)v);
+}
+
 // Adds a per-lane count, summed over the wavefront, to a work counter.
 __device__ __forceinline__ void count_wave(unsigned long long* stats, int idx, int v) {
 #pragma unroll
@@ -609,41 +624,75 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
         if (A.stats) {
             count_wave(A.stats, ASPH_C_DENS_CAND, ncand);
             count_wave(A.stats, ASPH_C_DENS_PAIRS, npair);
+            count_wave_max(A.stats, ASPH_C_DENS_WAVE_CAND, ncand);
         }
     }
 }
 
+// Whether a member's bricks take mode A (LDS) this launch: the drift bound and the halo width.
+template <int S, bool MOM>
+__device__ __forceinline__ bool member_mode_a(const DevParams& P, const Smem<S, MOM>& sm) {
+    return sm.modeA && P.nx >= BS + 2 * margin_of<S>(P, sm.delta[0]) &&
+           P.ny >= BS + 2 * margin_of<S>(P, sm.delta[1]);
+}
+
+// Owned range of work item `it` (brick it.x, chunk it.y): a brick's listed particles are split
+// into equal chunks of at most CHUNK.
+__device__ __forceinline__ void item_range(const uint32_t* __restrict__ cstart, uint32_t k0, uint32_t kb, int chunk,
+                                           uint32_t& o0, uint32_t& o1) {
+    const uint32_t b0 = cstart[k0 + kb], b1 = cstart[k0 + kb + (1u << NB)];
+    const uint32_t n = b1 - b0, ni = (n + CHUNK - 1) / CHUNK;
+    const uint32_t q = ni ? (n + ni - 1) / ni : 0u;
+    o0 = b0 + (uint32_t)chunk * q;
+    o1 = min(b1, o0 + q);
+}
+
+// Workgroup b plays up to two roles: mode B for brick b (members not in mode A), and mode A for
+// work item b (members in mode A). Every particle is computed by exactly one of them.
 template <int S>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_density_eos_brick(
-        const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, DensArgs A) {
+        const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
+        const uint32_t* __restrict__ nitems, int nbblk, int n_members, DensArgs A) {
     __shared__ Smem<S, false> sm;
     const int tid = threadIdx.x;
-    const BlockMap bm = bmap[blockIdx.x];
+    const int b = blockIdx.x;
+    if (b < n_members && tid < 3 * 64) A.maxv[b * 192 + tid] = 0u;   // max|v| slots of member b
+    prologue<S, false>(A.disp, A.disp_slots, sm);
+    if (b < nbblk) {  // mode B: particle per thread by index within the member
+        const BlockMap bm = bmap[b];
+        const MemberDesc& MD = mem[bm.m];
+        const DevParams& P = MD.P;
+        if (!member_mode_a<S, false>(P, sm)) {
+            count_wg(A.stats, ASPH_C_DENS_MODEB);
+            DensConst K;
+            K.p0 = MD.p0;
+            K.auto_f = auto_factor(P, A.dt);
+            sfc_params(P, A.clock ? A.clock[0] : A.simt, K.Ts, K.qs_sfc);
+            int il = bm.ord * BT + tid;
+            if (il < P.N) {
+                int i = MD.p0 + il;
+                float4 pi = A.pos[i];
+                float S1, S2;
+                dens_sum_global<S>(P, MD.k0, pi, A.pos, A.rho_old, A.cstart, S1, S2);
+                dens_epilogue(P, A, K, i, pi, S1, S2);
+            }
+        }
+    }
+    if ((uint32_t)b >= *nitems) return;
+    const int2 it = items[b];
+    const BlockMap bm = bmap[it.x];
     const MemberDesc& MD = mem[bm.m];
     const DevParams& P = MD.P;
+    if (!member_mode_a<S, false>(P, sm)) return;
     const uint32_t k0 = MD.k0;
-    if (bm.ord == 0 && tid < 3 * 64) A.maxv[bm.m * 192 + tid] = 0u;   // this member's max|v| slots
     DensConst K;
     K.p0 = MD.p0;
     K.auto_f = auto_factor(P, A.dt);
     sfc_params(P, A.clock ? A.clock[0] : A.simt, K.Ts, K.qs_sfc);
-    prologue<S, false>(P, A.disp, A.disp_slots, sm);
-    if (!sm.modeA) {  // mode B: particle per thread by index within the member
-        count_wg(A.stats, ASPH_C_DENS_MODEB);
-        int il = bm.ord * BT + tid;
-        if (il < P.N) {
-            int i = MD.p0 + il;
-            float4 pi = A.pos[i];
-            float S1, S2;
-            dens_sum_global<S>(P, k0, pi, A.pos, A.rho_old, A.cstart, S1, S2);
-            dens_epilogue(P, A, K, i, pi, S1, S2);
-        }
-        return;
-    }
-    // owned: the particles listed in the brick's 512 cells, one contiguous range
     const uint32_t kb = (uint32_t)bm.base;
-    const uint32_t o0 = A.cstart[k0 + kb], o1 = A.cstart[k0 + kb + (1u << NB)];
-    if (o0 == o1) return;
+    uint32_t o0, o1;
+    item_range(A.cstart, k0, kb, it.y, o0, o1);
+    if (o0 >= o1) return;
     const BrickGeo g = brick_geo<S, false>(P, sm, kb);
     plan<S, false>(P, k0, g, A.cstart, sm);
     if (sm.nslab == 0) {
@@ -699,14 +748,15 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
         auto flush = [&]() {
             for (int k = 0; wave_any(k < qn); ++k) {
                 if (k < qn) {
-                    int p = sm.q[k][tid];
-                    float4 pj = sm.p[p];
-                    float4 vj = A.vel[sm.j[p]];   // (vx, vy, vz, p'_j)
+                    float4 pj = sm.p[sm.q[k][tid]];
+                    const uint32_t jg = __float_as_uint(pj.w);
+                    float4 vj = A.vel[jg];                 // (vx, vy, vz, p'_j)
+                    float rhoj = fmaxf(A.rho[jg], 1e-3f);
                     float rx, ry, rz;
                     diff(pj, rx, ry, rz);
                     float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
                     float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
-                    mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, pj.w, vj, vj.w, M);
+                    mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, rhoj, vj, vj.w, M);
                 }
             }
             npair += qn;
@@ -734,6 +784,7 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
         if (A.stats) {
             count_wave(A.stats, ASPH_C_MOM_CAND, ncand);
             count_wave(A.stats, ASPH_C_MOM_PAIRS, npair);
+            count_wave_max(A.stats, ASPH_C_MOM_WAVE_CAND, ncand);
         }
     }
 }
@@ -755,33 +806,43 @@ __device__ __forceinline__ void reduce_maxv(uint32_t* maxv, float mx, float my, 
 
 template <int S>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_momentum_kdk_brick(
-        const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, MomArgs A) {
+        const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
+        const uint32_t* __restrict__ nitems, int nbblk, MomArgs A) {
     __shared__ Smem<S, true> sm;
     const int tid = threadIdx.x;
-    const BlockMap bm = bmap[blockIdx.x];
+    const int b = blockIdx.x;
+    prologue<S, true>(A.disp, A.disp_slots, sm);
+    if (b < nbblk) {
+        const BlockMap bm = bmap[b];
+        const MemberDesc& MD = mem[bm.m];
+        const DevParams& P = MD.P;
+        if (!member_mode_a<S, true>(P, sm)) {
+            count_wg(A.stats, ASPH_C_MOM_MODEB);
+            float mx = 0.f, my = 0.f, mz = 0.f;
+            int il = bm.ord * BT + tid;
+            if (il < P.N) {
+                int i = MD.p0 + il;
+                float4 pi = A.pos[i], vi = A.vel[i], ti = A.thq[i];
+                float rhoi = fmaxf(A.rho[i], 1e-3f), pri = A.prs[i];
+                MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+                mom_sum_global<S>(P, MD.k0, pi, vi, rhoi, pri, A.pos, A.vel, A.rho, A.prs, A.cstart, M);
+                mom_epilogue(P, A, i, pi, vi, ti, pri, M, mx, my, mz);
+            }
+            if (A.mode) reduce_maxv(A.maxv + bm.m * 192, mx, my, mz);
+        }
+    }
+    if ((uint32_t)b >= *nitems) return;
+    const int2 it = items[b];
+    const BlockMap bm = bmap[it.x];
     const MemberDesc& MD = mem[bm.m];
     const DevParams& P = MD.P;
+    if (!member_mode_a<S, true>(P, sm)) return;
     const uint32_t k0 = MD.k0;
-    uint32_t* maxv = A.maxv + bm.m * 192;
-    float mx = 0.f, my = 0.f, mz = 0.f;
-    prologue<S, true>(P, A.disp, A.disp_slots, sm);
-    if (!sm.modeA) {
-        count_wg(A.stats, ASPH_C_MOM_MODEB);
-        int il = bm.ord * BT + tid;
-        if (il < P.N) {
-            int i = MD.p0 + il;
-            float4 pi = A.pos[i], vi = A.vel[i], ti = A.thq[i];
-            float rhoi = fmaxf(A.rho[i], 1e-3f), pri = A.prs[i];
-            MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-            mom_sum_global<S>(P, k0, pi, vi, rhoi, pri, A.pos, A.vel, A.rho, A.prs, A.cstart, M);
-            mom_epilogue(P, A, i, pi, vi, ti, pri, M, mx, my, mz);
-        }
-        if (A.mode) reduce_maxv(maxv, mx, my, mz);
-        return;
-    }
     const uint32_t kb = (uint32_t)bm.base;
-    const uint32_t o0 = A.cstart[k0 + kb], o1 = A.cstart[k0 + kb + (1u << NB)];
-    if (o0 == o1) return;
+    uint32_t o0, o1;
+    item_range(A.cstart, k0, kb, it.y, o0, o1);
+    if (o0 >= o1) return;
+    float mx = 0.f, my = 0.f, mz = 0.f;
     const BrickGeo g = brick_geo<S, true>(P, sm, kb);
     plan<S, true>(P, k0, g, A.cstart, sm);
     if (sm.nslab == 0) {
@@ -796,38 +857,123 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     } else {
         count_wg(A.stats, sm.nslab == 1 ? ASPH_C_MOM_ONE_SLAB : ASPH_C_MOM_SLABBED);
         count_wg(A.stats, ASPH_C_MOM_STAGINGS, sm.nslab * (int)((o1 - o0 + BT - 1) / BT));
-        const float* rho = A.rho;
         auto rec = [&](Smem<S, true>& s, uint32_t slot, uint32_t jg, float4 pj) {
-            s.p[slot] = make_float4(pj.x, pj.y, pj.z, fmaxf(rho[jg], 1e-3f));
-            s.j[slot] = jg;
+            s.p[slot] = make_float4(pj.x, pj.y, pj.z, __uint_as_float(jg));
         };
         if (g.wraps) mom_brick<S, true>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
         else mom_brick<S, false>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
     }
-    if (A.mode) reduce_maxv(maxv, mx, my, mz);
+    if (A.mode) reduce_maxv(A.maxv + bm.m * 192, mx, my, mz);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Work items of the brick kernels: every non-empty brick is split into chunks of at most CHUNK
+// listed particles, and the chunks are ordered by the brick's estimated cost (chunk size x brick
+// occupancy, 4 bins per octave), heaviest first, so that the long workgroups of dense regions
+// start early instead of forming the tail of the launch. Counting sort: count, scan, fill.
+// ---------------------------------------------------------------------------------------------
+constexpr int ITEM_BINS = 128;
+
+// Per-brick item count and cost bin; the bin histogram is aggregated in LDS first (one global
+// atomic per bin and workgroup: most bricks share a bin).
+__global__ __launch_bounds__(256) void k_items_count(const MemberDesc* __restrict__ mem,
+                                                     const BlockMap* __restrict__ bmap, int nbblk,
+                                                     const uint32_t* __restrict__ cstart,
+                                                     uint32_t* __restrict__ meta, uint32_t* __restrict__ hist) {
+    __shared__ uint32_t lh[ITEM_BINS];
+    const int t = threadIdx.x;
+    if (t < ITEM_BINS) lh[t] = 0u;
+    __syncthreads();
+    const int b = blockIdx.x * 256 + t;
+    if (b < nbblk) {
+        const BlockMap bm = bmap[b];
+        const uint32_t k0 = mem[bm.m].k0, kb = (uint32_t)bm.base;
+        const uint32_t n = cstart[k0 + kb + (1u << NB)] - cstart[k0 + kb];
+        const uint32_t ni = (n + CHUNK - 1) / CHUNK;
+        int bin = 0;
+        if (ni) {
+            const float cost = (float)min(n, (uint32_t)CHUNK) * (float)n;
+            const int lg = (int)(4.0f * log2f(fmaxf(cost, 1.0f)));
+            bin = ITEM_BINS - 1 - min(lg, ITEM_BINS - 1);
+            atomicAdd(&lh[bin], ni);
+        }
+        meta[b] = (ni << 8) | (uint32_t)bin;
+    }
+    __syncthreads();
+    if (t < ITEM_BINS && lh[t]) atomicAdd(&hist[t], lh[t]);
+}
+
+// One workgroup of ITEM_BINS threads: bin starts (heaviest bin first) into hist[ITEM_BINS + k],
+// the item count, and the counts cleared for the next build.
+__global__ __launch_bounds__(ITEM_BINS) void k_items_scan(uint32_t* __restrict__ hist, uint32_t* __restrict__ nitems) {
+    __shared__ uint32_t v[ITEM_BINS];
+    const int t = threadIdx.x;
+    v[t] = hist[t];
+    hist[t] = 0u;
+    __syncthreads();
+    for (int o = 1; o < ITEM_BINS; o <<= 1) {
+        uint32_t add = t >= o ? v[t - o] : 0u;
+        __syncthreads();
+        v[t] += add;
+        __syncthreads();
+    }
+    hist[ITEM_BINS + t] = t ? v[t - 1] : 0u;
+    if (t == ITEM_BINS - 1) *nitems = v[t];
+}
+
+// Writes each brick's items at its bin's cursor: positions are reserved per workgroup and bin
+// with one global atomic, and handed out within the workgroup through LDS atomics.
+__global__ __launch_bounds__(256) void k_items_fill(int nbblk, const uint32_t* __restrict__ meta,
+                                                    uint32_t* __restrict__ hist, int2* __restrict__ items) {
+    __shared__ uint32_t lh[ITEM_BINS], base[ITEM_BINS];
+    const int t = threadIdx.x;
+    if (t < ITEM_BINS) lh[t] = 0u;
+    __syncthreads();
+    const int b = blockIdx.x * 256 + t;
+    uint32_t m = 0u, local = 0u;
+    if (b < nbblk) {
+        m = meta[b];
+        if (m >> 8) local = atomicAdd(&lh[m & 0xffu], m >> 8);
+    }
+    __syncthreads();
+    if (t < ITEM_BINS && lh[t]) base[t] = atomicAdd(&hist[ITEM_BINS + t], lh[t]);
+    __syncthreads();
+    const uint32_t ni = m >> 8;
+    if (!ni) return;
+    const uint32_t pos = base[m & 0xffu] + local;
+    for (uint32_t k = 0; k < ni; ++k) items[pos + k] = make_int2(b, (int)k);
 }
 
 }  // namespace
 
 int brick_bits() { return NB; }
+int brick_item_bins() { return ITEM_BINS; }
+int brick_item_capacity(int nbblk, int64_t n) { return nbblk + (int)((n + CHUNK - 1) / CHUNK); }
+
+void launch_brick_items(const Launch& L, hipStream_t s, const uint32_t* cstart) {
+    const int g = (L.nbblk + 255) / 256;
+    k_items_count<<<g, 256, 0, s>>>(L.mem, L.bmap, L.nbblk, cstart, L.item_meta, L.item_hist);
+    k_items_scan<<<1, ITEM_BINS, 0, s>>>(L.item_hist, L.nitems);
+    k_items_fill<<<g, 256, 0, s>>>(L.nbblk, L.item_meta, L.item_hist, L.items);
+}
 
 void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A) {
-    dim3 g(L.nbblk);
+    dim3 g(L.nitem_max);
     switch (S) {
-        case 1: k_density_eos_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
-        case 2: k_density_eos_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
-        case 3: k_density_eos_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
-        default: k_density_eos_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
+        case 1: k_density_eos_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, A); break;
+        case 2: k_density_eos_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, A); break;
+        case 3: k_density_eos_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, A); break;
+        default: k_density_eos_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, A); break;
     }
 }
 
 void launch_momentum_kdk(const Launch& L, int S, hipStream_t s, const MomArgs& A) {
-    dim3 g(L.nbblk);
+    dim3 g(L.nitem_max);
     switch (S) {
-        case 1: k_momentum_kdk_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
-        case 2: k_momentum_kdk_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
-        case 3: k_momentum_kdk_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
-        default: k_momentum_kdk_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, A); break;
+        case 1: k_momentum_kdk_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, A); break;
+        case 2: k_momentum_kdk_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, A); break;
+        case 3: k_momentum_kdk_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, A); break;
+        default: k_momentum_kdk_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, A); break;
     }
 }
 

@@ -217,17 +217,19 @@ def test_clumped_overflow_fallback():
 
 
 def _compressed(a):
-    """The moist configuration on 32 x 32 x 24 cells with every particle position scaled by 1/a
-    towards the origin: number density a^3 in a region wider than a brick halo."""
+    """The moist configuration on 32 x 32 x 24 cells with every particle coordinate divided by
+    a = (ax, ay, az) towards the origin: number density ax*ay*az in a region wider than a brick
+    halo."""
     cfg, ic, dt = small_config("moist", (32, 32, 24))
     st = s.ic_generate(cfg, ic)
-    st.pos[:, :3] = (st.pos[:, :3] / np.float32(a)).astype(np.float32)
+    st.pos[:, :3] = (st.pos[:, :3] / np.asarray(a, np.float32)).astype(np.float32)
     return cfg, dt, st
 
 
 # (compression, the path the densest bricks must take): a halo beyond the LDS capacity is
-# streamed through it in z-slabs; a brick with one halo layer beyond it takes the global loop
-SPLITS = [(1.2, "slabbed"), (1.7, "slabbed"), (3.5, "global")]
+# streamed through it in z-slabs; a brick with one halo layer beyond it (here ~39 particles per
+# cell) takes the global-memory loop
+SPLITS = [((1.2, 1.2, 1.2), "slabbed"), ((1.7, 1.7, 1.7), "slabbed"), ((2.2, 2.2, 8.0), "global")]
 
 
 @pytest.mark.parametrize("a,path", SPLITS)
