@@ -83,6 +83,69 @@ def copy_probe(mib):
     log(f"[bench] copy probe: 3 x {mib} MiB device-to-device")
 
 
+def ensemble_bench(n_members, steps, seed=5, sequential=True):
+    """The batched ensemble against sequential plain runs of the same members.
+
+    Members from the library's ensemble preset (log-uniform 8,192-131,072 particles, 250 m
+    lattice, moist, per-member noise seed), all re-sorted every step so both runs follow the
+    same schedule. Batched: one handle, one launch per stage for all members, steps replayed
+    from a captured hipGraph with no host synchronisation. Sequential: one handle per member,
+    the reference loop (asph_run) with its per-step max|v| readback. After one untimed warm-up
+    step on both sides, `steps` steps are timed; the final states must agree bit for bit."""
+    import numpy as np
+
+    import sph_cuda_amd as s
+    from sph_cuda_amd import _lib as L
+
+    cfgs, ics, dt, _ = s.ensemble_preset(n_members, seed)
+    for c in cfgs:
+        c.flags |= L.F_SORT_EVERY_STEP
+    states = [s.ic_generate(c, ic) for c, ic in zip(cfgs, ics)]
+    ntot = sum(c.n for c in cfgs)
+    log(f"[bench] ensemble: {n_members} members, {ntot} particles, dt {dt}")
+
+    ens = s.Sim(cfgs)
+    ens.set_state(s.concat_states(states))
+    ens.run_graph(1, dt, sort_every=1)
+    ens.synchronize()
+    t0 = time.perf_counter()
+    ens.run_graph(steps, dt, sort_every=1)
+    ens.synchronize()
+    wall_b = time.perf_counter() - t0
+    final = ens.get_state()
+    log(f"[bench] batched: {steps} steps in {wall_b:.3f} s")
+
+    wall_s, exact = 0.0, 0
+    for m, (c, st) in enumerate(zip(cfgs, states) if sequential else []):
+        with s.Sim(c) as one:
+            one.set_state(st)
+            rc, _ = one.run(max_steps=1, dt=dt, fixed_dt=True)
+            one.synchronize()
+            t0 = time.perf_counter()
+            rc2, _ = one.run(max_steps=steps, dt=dt, fixed_dt=True)
+            one.synchronize()
+            wall_s += time.perf_counter() - t0
+            if rc or rc2:
+                raise SystemExit(f"member {m}: run aborted")
+            o = one.get_state()
+            a, n = ens.member_range(m)
+            exact += int(all(np.array_equal(getattr(final, f)[a:a + n], getattr(o, f))
+                             for f in ("pos", "vel", "thq", "rho")))
+    log(f"[bench] sequential: {wall_s:.3f} s; members bit-identical: {exact}/{n_members}")
+    return {
+        "workload": f"{n_members}-member ensemble, {ntot} particles (8,192-131,072 per member, "
+                    f"250 m lattice, nz 32), moist, dt {dt} s, re-sort every step",
+        "members": n_members, "particles": ntot, "steps": steps,
+        "value": ntot * steps / wall_b, "unit": "particle-steps/s",
+        "ms_per_step": wall_b / steps * 1e3,
+        "sequential_value": ntot * steps / wall_s if sequential else None,
+        "sequential_ms_per_step": wall_s / steps * 1e3 if sequential else None,
+        "speedup_vs_sequential": wall_s / wall_b if sequential else None,
+        "members_bit_identical": exact if sequential else None,
+        "hbm_frac_step_alg": (B_STEP + B_SORT) * ntot * steps / wall_b / 1e9 / HBM_PEAK_GBS,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,6 +155,11 @@ def main():
     ap.add_argument("--n", type=int, default=0, help="override lattice edge (cubic)")
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble leg")
+    ap.add_argument("--ensemble-members", type=int, default=256)
+    ap.add_argument("--ensemble-steps", type=int, default=240)
+    ap.add_argument("--ensemble-only", action="store_true",
+                    help="run only the batched leg of the ensemble (profiling)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size 3 times (device to device) as "
                          "the byte reference for hardware traffic counters")
@@ -103,6 +171,10 @@ def main():
 
     import sph_cuda_amd as s
 
+    if args.ensemble_only:
+        print(json.dumps(ensemble_bench(args.ensemble_members, args.ensemble_steps, sequential=False)),
+              flush=True)
+        return
     cfg, ic, dt, _ = s.preset(args.preset)
     if args.n:
         cfg.nx = cfg.ny = cfg.nz = args.n
@@ -174,7 +246,11 @@ def main():
             "traffic_GBps_at_measured_time": (traffic / avg_s / 1e9) if traffic else None,
         },
         "cpu_baseline": None,
+        "batched": None,
     }
+    if not args.no_ensemble:
+        del sim
+        out["batched"] = ensemble_bench(args.ensemble_members, args.ensemble_steps)
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
     if not args.no_cpu_baseline:

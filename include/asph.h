@@ -145,6 +145,14 @@ int asph_config_reference(asph_config* cfg, int32_t nx, int32_t ny, int32_t nz);
 int asph_config_preset(int32_t which, asph_config* cfg, asph_ic_spec* ic, float* dt,
                        int32_t* steps);
 
+/* The batched-ensemble workload: n members of the preset-5 template (250 m isotropic lattice,
+ * moist, white theta noise 0.1 K), member m with N_m log-uniform in [8192, 131072] from
+ * splitmix64(splitmix64(seed) ^ m), nz = 32, nx = ny = round(sqrt(N_m / 32)), and that hash as
+ * its noise seed. Fills cfgs[n] and (if non-NULL) ics[n], dt and steps. No reference
+ * counterpart: the reference runs one domain per process. */
+int asph_ensemble_preset(int32_t n, uint64_t seed, asph_config* cfgs, asph_ic_spec* ics, float* dt,
+                         int32_t* steps);
+
 /* Validates cfg (Morton limit 1024 cells per axis, asph_h100.cu:785-788; stencil fits). */
 int asph_config_check(const asph_config* cfg);
 

@@ -108,6 +108,7 @@ SIGNATURES = {
     "asph_config_reference": (C.c_int, [C.POINTER(Config), C.c_int32, C.c_int32, C.c_int32]),
     "asph_config_preset": (C.c_int, [C.c_int32, C.POINTER(Config), C.POINTER(ICSpec), FP, I32P]),
     "asph_config_check": (C.c_int, [C.POINTER(Config)]),
+    "asph_ensemble_preset": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(Config), C.POINTER(ICSpec), FP, I32P]),
     "asph_ic_generate": (C.c_int, [C.POINTER(Config), C.POINTER(ICSpec), FP, FP, FP, FP]),
     "asph_create": (C.c_int, [C.POINTER(Config), P, C.POINTER(P)]),
     "asph_destroy": (C.c_int, [P]),

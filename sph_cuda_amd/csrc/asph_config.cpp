@@ -298,6 +298,34 @@ int asph_config_preset(int32_t which, asph_config* c, asph_ic_spec* ic, float* d
     return ASPH_OK;
 }
 
+int asph_ensemble_preset(int32_t n, uint64_t seed, asph_config* cfgs, asph_ic_spec* ics, float* dt,
+                         int32_t* steps) {
+    if (n < 1 || !cfgs) return fail("need n >= 1 and a config array");
+    asph_config tmpl;
+    asph_ic_spec ic;
+    float pdt = 0.f;
+    int32_t pst = 0;
+    if (asph_config_preset(5, &tmpl, &ic, &pdt, &pst) != ASPH_OK) return ASPH_ERR;
+    for (int32_t m = 0; m < n; ++m) {
+        // member m: particle count log-uniform in [8192, 131072], lattice nz = 32 and
+        // nx = ny = round(sqrt(N / 32)); its own noise seed
+        const uint64_t hm = splitmix64(splitmix64(seed) ^ (uint64_t)m);
+        const double u = (double)(hm >> 11) * (1.0 / 9007199254740992.0);
+        const double N = std::exp(std::log(8192.0) + u * (std::log(131072.0) - std::log(8192.0)));
+        const int nxy = (int)std::lround(std::sqrt(N / 32.0));
+        cfgs[m] = tmpl;
+        cfgs[m].nx = cfgs[m].ny = nxy;
+        cfgs[m].nz = 32;
+        if (ics) {
+            ics[m] = ic;
+            ics[m].seed = hm;
+        }
+    }
+    if (dt) *dt = pdt;
+    if (steps) *steps = pst;
+    return ASPH_OK;
+}
+
 int asph_config_check(const asph_config* c) {
     if (!c) return fail("null config");
     if (c->nx < 1 || c->ny < 1 || c->nz < 1) return fail("grid must be positive");

@@ -194,6 +194,11 @@ int create_impl(const asph_config* cfgs, int n, void* stream, asph_t** out) {
     rc |= h->alloc(&nitems, 1);
     rc |= h->alloc(&item_hist, (size_t)2 * brick_item_bins());
     rc |= h->alloc(&item_meta, h->bmap_h.size());
+    const uint32_t gapcap = cell_gap_capacity(h->nkeys);
+    uint4* gaps = nullptr;
+    uint32_t* ngaps = nullptr;
+    rc |= h->alloc(&gaps, gapcap);
+    rc |= h->alloc(&ngaps, 1);
     if (rc != ASPH_OK) { std::string e = g_err; asph_destroy(h); g_err = e; return ASPH_ERR; }
     hipError_t e = hipMemcpy(h->mem_d, h->mem_h.data(), h->mem_h.size() * sizeof(MemberDesc), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->pmap_d, h->pmap_h.data(), h->pmap_h.size() * sizeof(BlockMap), hipMemcpyHostToDevice);
@@ -206,6 +211,7 @@ int create_impl(const asph_config* cfgs, int n, void* stream, asph_t** out) {
     h->L.npblk = (int)h->pmap_h.size(); h->L.nbblk = (int)h->bmap_h.size(); h->L.n_members = n;
     h->L.items = items; h->L.nitems = nitems; h->L.item_hist = item_hist; h->L.item_meta = item_meta;
     h->L.nitem_max = nitem_max;
+    h->L.gaps = gaps; h->L.ngaps = ngaps; h->L.gapcap = gapcap;
     *out = h;
     return ASPH_OK;
 }
@@ -230,7 +236,7 @@ void do_sort(asph_t* h, bool launch) {
         const bool ga = (h->cfgs[0].flags & ASPH_F_GATHER_ACC) != 0;
         launch_gather(h->stream, (int)h->N, h->perm, h->pos, h->pos2, h->vel, h->vel2, h->thq,
                       h->thq2, h->rho, h->rho2, ga ? h->acc : nullptr, ga ? h->acc2 : nullptr);
-        launch_cell_start(h->stream, (int)h->N, h->nkeys, h->keys_sorted, h->cstart);
+        launch_cell_start(h->L, h->stream, (int)h->N, h->nkeys, h->keys_sorted, h->cstart);
         launch_brick_items(h->L, h->stream, h->cstart);
         launch_step_begin(h->stream, h->disp, h->disp_slots, 1, h->clock, 2, 0.0, 0.f);
     } else {

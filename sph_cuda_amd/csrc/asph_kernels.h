@@ -20,6 +20,10 @@ struct Launch {
     uint32_t* item_hist;     // [2 * brick_item_bins()] counting-sort scratch (zero between builds)
     uint32_t* item_meta;     // [nbblk] scratch
     int nitem_max;           // grid of the brick kernels
+    // runs of empty keys longer than cell_gap_serial(), filled by whole workgroups
+    uint4* gaps;             // [gapcap]: (first key, last key, value)
+    uint32_t* ngaps;         // reset by the keys kernel of each sort
+    uint32_t gapcap;
 };
 
 struct DensArgs {
@@ -79,7 +83,9 @@ void launch_keys(const Launch& L, hipStream_t s, const float4* pos, uint32_t* ke
 void launch_gather(hipStream_t s, int N, const uint32_t* perm, const float4* pA, float4* pB,
                    const float4* vA, float4* vB, const float4* tA, float4* tB, const float* rA,
                    float* rB, const float4* aA, float4* aB);
-void launch_cell_start(hipStream_t s, int N, uint32_t nkeys, const uint32_t* keys, uint32_t* cstart);
+uint32_t cell_gap_capacity(uint32_t nkeys);
+void launch_cell_start(const Launch& L, hipStream_t s, int N, uint32_t nkeys, const uint32_t* keys,
+                       uint32_t* cstart);
 void launch_clamp_vel(const Launch& L, hipStream_t s, float4* vel, float vmax, float cs_frac);
 void launch_audit(const Launch& L, hipStream_t s, const float4* pos, const float4* vel,
                   const float4* thq, const int64_t* rain, const int64_t* dep, const int64_t* precip,

@@ -166,7 +166,8 @@ __global__ __launch_bounds__(TPB) void k_rain_merge(int n, int64_t* __restrict__
 __global__ __launch_bounds__(TPB) void k_keys(const MemberDesc* __restrict__ mem,
                                               const BlockMap* __restrict__ pmap,
                                               const float4* __restrict__ pos,
-                                              uint32_t* __restrict__ keys) {
+                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ ngaps) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *ngaps = 0u;   // gap list of this sort's cell table
     const BlockMap bm = pmap[blockIdx.x];
     const MemberDesc& M = mem[bm.m];
     const DevParams& P = M.P;
@@ -197,20 +198,43 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const uint32_t* __restric
 }
 
 // Cell start table from the sorted keys: cstart[k] = first i with key_i >= k.
+// Cell start table from the sorted keys: cstart[q] = index of the first particle with key >= q.
+// The particle that starts a run of equal keys (and the end sentinel) owns the run of empty keys
+// before it: runs of up to GAP_SERIAL keys it fills itself; longer ones (sparse key spaces:
+// ensemble members, boxes that are not powers of two) go to a list that whole workgroups fill.
+constexpr uint32_t GAP_SERIAL = 256;
+
 __global__ __launch_bounds__(TPB) void k_cell_start(int N, uint32_t nkeys,
                                                     const uint32_t* __restrict__ keys,
-                                                    uint32_t* __restrict__ cstart) {
-    int i = blockIdx.x * TPB + threadIdx.x;
-    if (i < N && i > 0) {
-        uint32_t k = keys[i], kp = keys[i - 1];
-        for (uint32_t q = kp + 1; q <= k; ++q) cstart[q] = (uint32_t)i;
+                                                    uint32_t* __restrict__ cstart, uint4* __restrict__ gaps,
+                                                    uint32_t* __restrict__ ngaps, uint32_t gapcap) {
+    const int i = blockIdx.x * TPB + threadIdx.x;
+    if (i > N) return;
+    uint32_t lo, hi;
+    if (i == 0) { lo = 0u; hi = keys[0]; }
+    else if (i == N) { lo = keys[N - 1] + 1u; hi = nkeys; }
+    else {
+        const uint32_t k = keys[i], kp = keys[i - 1];
+        if (k == kp) return;
+        lo = kp + 1u; hi = k;
     }
-    // head [0, key_0] -> 0 and tail (key_{N-1}, nkeys] -> N, grid-strided
-    uint32_t k0 = keys[0], kl = keys[N - 1];
-    uint32_t stride = gridDim.x * TPB;
-    for (uint32_t q = (uint32_t)(blockIdx.x * TPB + threadIdx.x); q <= k0; q += stride) cstart[q] = 0u;
-    for (uint32_t q = kl + 1 + (uint32_t)(blockIdx.x * TPB + threadIdx.x); q <= nkeys; q += stride)
-        cstart[q] = (uint32_t)N;
+    if (lo > hi) return;
+    if (hi - lo < GAP_SERIAL) {
+        for (uint32_t q = lo; q <= hi; ++q) cstart[q] = (uint32_t)i;
+        return;
+    }
+    const uint32_t g = atomicAdd(ngaps, 1u);
+    if (g < gapcap) gaps[g] = make_uint4(lo, hi, (uint32_t)i, 0u);
+    else for (uint32_t q = lo; q <= hi; ++q) cstart[q] = (uint32_t)i;   // not reached: gapcap bounds the runs
+}
+
+__global__ __launch_bounds__(TPB) void k_cell_gaps(const uint4* __restrict__ gaps, const uint32_t* __restrict__ ngaps,
+                                                   uint32_t gapcap, uint32_t* __restrict__ cstart) {
+    const uint32_t n = min(*ngaps, gapcap);
+    for (uint32_t g = blockIdx.x; g < n; g += gridDim.x) {
+        const uint4 r = gaps[g];
+        for (uint32_t q = r.x + threadIdx.x; q <= r.y; q += TPB) cstart[q] = r.z;
+    }
 }
 
 // kernel_clamp_vel, asph_h100.cu:669-675: |v| <= vmax_abs, or <= cs_frac * cs of each member.
@@ -364,7 +388,7 @@ void launch_rain_merge(hipStream_t s, int n, int64_t* rain, int64_t* dep) {
 }
 
 void launch_keys(const Launch& L, hipStream_t s, const float4* pos, uint32_t* keys) {
-    k_keys<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, pos, keys);
+    k_keys<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, pos, keys, L.ngaps);
 }
 
 void launch_gather(hipStream_t s, int N, const uint32_t* perm, const float4* pA, float4* pB,
@@ -373,8 +397,13 @@ void launch_gather(hipStream_t s, int N, const uint32_t* perm, const float4* pA,
     k_gather<<<nblocks(N), TPB, 0, s>>>(N, perm, pA, pB, vA, vB, tA, tB, rA, rB, aA, aB);
 }
 
-void launch_cell_start(hipStream_t s, int N, uint32_t nkeys, const uint32_t* keys, uint32_t* cstart) {
-    k_cell_start<<<nblocks(N), TPB, 0, s>>>(N, nkeys, keys, cstart);
+// Disjoint runs longer than GAP_SERIAL keys: at most (nkeys + 1) / (GAP_SERIAL + 1) + 1 of them.
+uint32_t cell_gap_capacity(uint32_t nkeys) { return (nkeys + 1u) / (GAP_SERIAL + 1u) + 2u; }
+
+void launch_cell_start(const Launch& L, hipStream_t s, int N, uint32_t nkeys, const uint32_t* keys,
+                       uint32_t* cstart) {
+    k_cell_start<<<nblocks((int64_t)N + 1), TPB, 0, s>>>(N, nkeys, keys, cstart, L.gaps, L.ngaps, L.gapcap);
+    k_cell_gaps<<<1024, TPB, 0, s>>>(L.gaps, L.ngaps, L.gapcap, cstart);
 }
 
 void launch_clamp_vel(const Launch& L, hipStream_t s, float4* vel, float vmax, float cs_frac) {

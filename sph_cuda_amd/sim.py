@@ -30,6 +30,20 @@ def preset(which: int):
     return cfg, ic, float(dt.value), int(steps.value)
 
 
+def ensemble_preset(n: int, seed: int = 5):
+    """The batched-ensemble workload: ([Config] * n, [ICSpec] * n, dt, steps)."""
+    cfgs, ics = (L.Config * n)(), (L.ICSpec * n)()
+    dt, steps = C.c_float(), C.c_int32()
+    L.check(L.load().asph_ensemble_preset(n, seed, cfgs, ics, C.byref(dt), C.byref(steps)),
+            "asph_ensemble_preset")
+    return [c.copy() for c in cfgs], [ics[k] for k in range(n)], float(dt.value), int(steps.value)
+
+
+def concat_states(states) -> "State":
+    """Concatenates member states in member order (the ensemble layout)."""
+    return State(*[np.concatenate([getattr(x, f) for x in states]) for f in ("pos", "vel", "thq", "rho")])
+
+
 def reference_config(nx: int, ny: int, nz: int) -> L.Config:
     lib = L.load()
     cfg = L.Config()
