@@ -90,8 +90,9 @@ def ensemble_bench(n_members, steps, seed=5, sequential=True):
     lattice, moist, per-member noise seed), all re-sorted every step so both runs follow the
     same schedule. Batched: one handle, one launch per stage for all members, steps replayed
     from a captured hipGraph with no host synchronisation. Sequential: one handle per member,
-    the reference loop (asph_run) with its per-step max|v| readback. After one untimed warm-up
-    step on both sides, `steps` steps are timed; the final states must agree bit for bit."""
+    the reference loop (asph_run) with its per-step max|v| readback, default decomposition. After
+    two untimed warm-up steps on both sides (the batched side autotunes its brick decomposition
+    between them), `steps` steps are timed; the final states must agree bit for bit."""
     import numpy as np
 
     import sph_cuda_amd as s
@@ -107,6 +108,9 @@ def ensemble_bench(n_members, steps, seed=5, sequential=True):
     ens = s.Sim(cfgs)
     ens.set_state(s.concat_states(states))
     ens.run_graph(1, dt, sort_every=1)
+    tune = ens.autotune(reps=2)
+    log(f"[bench] ensemble autotune: {tune['best']} of {tune['trials']}")
+    ens.run_graph(1, dt, sort_every=1)        # captures the step graph for the chosen shape
     ens.synchronize()
     t0 = time.perf_counter()
     ens.run_graph(steps, dt, sort_every=1)
@@ -119,7 +123,7 @@ def ensemble_bench(n_members, steps, seed=5, sequential=True):
     for m, (c, st) in enumerate(zip(cfgs, states) if sequential else []):
         with s.Sim(c) as one:
             one.set_state(st)
-            rc, _ = one.run(max_steps=1, dt=dt, fixed_dt=True)
+            rc, _ = one.run(max_steps=2, dt=dt, fixed_dt=True)
             one.synchronize()
             t0 = time.perf_counter()
             rc2, _ = one.run(max_steps=steps, dt=dt, fixed_dt=True)
@@ -142,6 +146,7 @@ def ensemble_bench(n_members, steps, seed=5, sequential=True):
         "sequential_ms_per_step": wall_s / steps * 1e3 if sequential else None,
         "speedup_vs_sequential": wall_s / wall_b if sequential else None,
         "members_bit_identical": exact if sequential else None,
+        "tuning": tune["best"],
         "hbm_frac_step_alg": (B_STEP + B_SORT) * ntot * steps / wall_b / 1e9 / HBM_PEAK_GBS,
     }
 
@@ -190,6 +195,8 @@ def main():
         raise SystemExit(f"warmup aborted rc={rc}")
     sim.synchronize()
     log(f"[bench] warmup {args.warmup} steps done ({ws.sorts} sorts)")
+    tune = sim.autotune(reps=2)
+    log(f"[bench] autotune: {tune['best']} of {tune['trials']}")
 
     sim.set_profiling(True)
     sim.kernel_times()   # reset
@@ -238,6 +245,7 @@ def main():
             "kernel_ms_per_launch": {k: (v[0] / max(v[1], 1)) for k, v in kt.items()},
             "kernel_launches": {k: v[1] for k, v in kt.items()},
             "parallelism": "single GPU",
+            "tuning": tune["best"],
         },
         "roofline": {
             "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,

@@ -280,6 +280,33 @@ int asph_column_diag(asph_t* h, float* out4);
 int asph_write_dump(asph_t* h, const char* outdir, int32_t sx, int32_t sy, int32_t sz,
                     double simt, float dt, char* path_out, size_t path_cap);
 
+/* --- tuning of the neighbour kernels ----------------------------------------------------- */
+
+/* Shape of the brick decomposition (no reference counterpart). Each 8x8x8-cell brick's listed
+ * particles are split into equal work items of at most `chunk` particles (256..1024), one
+ * 512-thread workgroup each; order 1 launches the items heaviest first (estimated cost), order
+ * 0 in brick key order. Defaults: chunk 768, order 1. Changing it drops captured graphs. */
+typedef struct asph_tuning {
+    int32_t chunk;
+    int32_t order;
+} asph_tuning;
+
+#define ASPH_TUNE_MAX 16
+typedef struct asph_tune_report {
+    int32_t n;                       /* candidates timed */
+    asph_tuning cand[ASPH_TUNE_MAX];
+    double ms[ASPH_TUNE_MAX];        /* mean ms of density + momentum on the current state */
+    int32_t best;                    /* index of the chosen candidate */
+} asph_tune_report;
+
+int asph_set_tuning(asph_t* h, const asph_tuning* t);
+int asph_get_tuning(asph_t* h, asph_tuning* t);
+/* Times the two neighbour kernels for every candidate shape on the current sorted,
+ * force-evaluated state (reps launches each, after one warm-up), keeps the fastest (the
+ * heaviest-first order unless brick order is more than 3% faster) and reports all. The state
+ * (including acc, dthdt and max|v|) is unchanged. Allocates scratch for the duration. */
+int asph_autotune(asph_t* h, int32_t reps, asph_tune_report* rep);
+
 /* --- profiling --------------------------------------------------------------------------- */
 
 /* Kernel ids for asph_kernel_times. */

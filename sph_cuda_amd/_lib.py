@@ -92,6 +92,18 @@ class RunArgs(C.Structure):
     ]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("chunk", C.c_int32), ("order", C.c_int32)]
+
+
+TUNE_MAX = 16
+
+
+class TuneReport(C.Structure):
+    _fields_ = [("n", C.c_int32), ("cand", Tuning * TUNE_MAX), ("ms", C.c_double * TUNE_MAX),
+                ("best", C.c_int32)]
+
+
 P = C.c_void_p
 FP = C.POINTER(C.c_float)
 DP = C.POINTER(C.c_double)
@@ -150,6 +162,9 @@ SIGNATURES = {
     "asph_kernel_times": (C.c_int, [P, DP, I64P]),
     "asph_synchronize": (C.c_int, [P]),
     "asph_set_counters": (C.c_int, [P, C.c_int32]),
+    "asph_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+    "asph_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
+    "asph_autotune": (C.c_int, [P, C.c_int32, C.POINTER(TuneReport)]),
     "asph_get_counters": (C.c_int, [P, C.POINTER(C.c_uint64)]),
 }
 

@@ -50,7 +50,7 @@ struct asph_t {
     bool dep_pending = false;        // dep holds deposits not yet merged into the rain grid
     double simt = 0.0, since_sort = 1e9;
     float sort_dt = 0.5f;
-    // hipGraph cache: one executable per (buffer assignment, sort flag, dt)
+    // hipGraph cache: one executable per (buffer assignment, sort flag, dt); tuning changes drop it
     std::map<std::tuple<void*, void*, void*, void*, void*, void*, int, float>, hipGraphExec_t> graphs;
     // profiling
     bool prof = false;
@@ -211,6 +211,8 @@ int create_impl(const asph_config* cfgs, int n, void* stream, asph_t** out) {
     h->L.npblk = (int)h->pmap_h.size(); h->L.nbblk = (int)h->bmap_h.size(); h->L.n_members = n;
     h->L.items = items; h->L.nitems = nitems; h->L.item_hist = item_hist; h->L.item_meta = item_meta;
     h->L.nitem_max = nitem_max;
+    h->L.chunk = 768;
+    h->L.order = 1;
     h->L.gaps = gaps; h->L.ngaps = ngaps; h->L.gapcap = gapcap;
     *out = h;
     return ASPH_OK;
@@ -868,6 +870,137 @@ int asph_kernel_times(asph_t* h, double ms[ASPH_K_COUNT], int64_t counts[ASPH_K_
         h->ev_live[k].clear();
     }
     return ASPH_OK;
+}
+
+// --- tuning of the brick decomposition --------------------------------------------------------
+namespace {
+
+void drop_graphs(asph_t* h) {
+    (void)hipStreamSynchronize(h->stream);
+    for (auto& kv : h->graphs) (void)hipGraphExecDestroy(kv.second);
+    h->graphs.clear();
+}
+
+// Density without the EOS/microphysics epilogue into the scratch density buffer: reads the
+// state, writes nothing the step keeps.
+DensArgs trial_dens_args(asph_t* h, float dt) {
+    DensArgs A;
+    A.pos = h->pos; A.rho_old = h->rho; A.rho_new = h->rho2; A.thq = h->thq; A.prs = h->prs;
+    A.vel = h->vel; A.cstart = h->cstart; A.keys_sorted = h->keys_sorted; A.rain = h->rain;
+    A.dep = h->dep; A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
+    A.clock = nullptr; A.simt = h->simt; A.stats = nullptr;
+    A.dt = dt; A.bits = 0;
+    return A;
+}
+
+}  // namespace
+
+int asph_set_tuning(asph_t* h, const asph_tuning* t) {
+    if (!h || !t) return fail("null argument");
+    if (t->chunk < brick_chunk_min() || t->chunk > brick_chunk_max())
+        return fail("chunk %d outside [%d, %d]", t->chunk, brick_chunk_min(), brick_chunk_max());
+    if (t->order != 0 && t->order != 1) return fail("order must be 0 or 1");
+    if (t->chunk != h->L.chunk || t->order != h->L.order) drop_graphs(h);
+    h->L.chunk = t->chunk;
+    h->L.order = t->order;
+    if (h->sorted) {
+        launch_brick_items(h->L, h->stream, h->cstart);
+        LAUNCHCHK();
+    }
+    return ASPH_OK;
+}
+
+int asph_get_tuning(asph_t* h, asph_tuning* t) {
+    if (!h || !t) return fail("null argument");
+    t->chunk = h->L.chunk;
+    t->order = h->L.order;
+    return ASPH_OK;
+}
+
+int asph_autotune(asph_t* h, int32_t reps, asph_tune_report* rep) {
+    if (!h) return fail("null handle");
+    if (!h->sorted || !h->initialized) return fail("autotune needs a sorted, force-evaluated state");
+    if (reps < 1) reps = 1;
+    static const int32_t chunks[] = {256, 384, 512, 768, 1024};
+    std::vector<asph_tuning> cand;
+    for (int32_t o = 1; o >= 0; --o)
+        for (int32_t c : chunks) cand.push_back({c, o});
+    const size_t N = (size_t)h->N;
+    const size_t nmv = (size_t)192 * h->n_members();
+    float4* acc_save = nullptr;
+    float* dth_save = nullptr;
+    uint32_t* maxv_save = nullptr;
+    if (hipMalloc(&acc_save, N * sizeof(float4)) != hipSuccess) return fail("autotune: out of memory");
+    if (hipMalloc(&dth_save, N * sizeof(float)) != hipSuccess ||
+        hipMalloc(&maxv_save, nmv * sizeof(uint32_t)) != hipSuccess) {
+        (void)hipFree(acc_save);
+        (void)hipFree(dth_save);
+        return fail("autotune: out of memory");
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    const asph_tuning keep = {h->L.chunk, h->L.order};
+    drop_graphs(h);
+    int rc = ASPH_OK;
+    std::vector<double> ms(cand.size(), 0.0);
+    const float dt = 0.1f;   // enters only the skipped epilogue
+    hipError_t e = hipMemcpyAsync(acc_save, h->acc, N * sizeof(float4), hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dth_save, h->dthdt, N * sizeof(float), hipMemcpyDeviceToDevice, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(maxv_save, h->maxv, nmv * sizeof(uint32_t), hipMemcpyDeviceToDevice, h->stream);
+    for (size_t k = 0; k < cand.size() && e == hipSuccess; ++k) {
+        h->L.chunk = cand[k].chunk;
+        h->L.order = cand[k].order;
+        launch_brick_items(h->L, h->stream, h->cstart);
+        for (int r = -1; r < reps; ++r) {   // r = -1: warm-up
+            if (r == 0) (void)hipEventRecord(e0, h->stream);
+            launch_density_eos(h->L, h->S, h->stream, trial_dens_args(h, dt));
+            launch_momentum_kdk(h->L, h->S, h->stream, mom_args(h, dt, 0));
+        }
+        (void)hipEventRecord(e1, h->stream);
+        e = hipEventSynchronize(e1);
+        float t = 0.f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        ms[k] = (double)t / reps;
+    }
+    // fastest shape; heaviest-first order is kept unless brick order beats it by more than 3%:
+    // on a uniform state the two tie, and only the heaviest-first order protects the launch
+    // from the tail of dense regions that develop later
+    size_t best = 0, best1 = 0;
+    for (size_t k = 1; k < cand.size(); ++k) {
+        if (ms[k] < ms[best]) best = k;
+        if (cand[k].order == 1 && (cand[best1].order != 1 || ms[k] < ms[best1])) best1 = k;
+    }
+    if (cand[best].order == 0 && cand[best1].order == 1 && ms[best1] <= 1.03 * ms[best]) best = best1;
+    if (e != hipSuccess) {
+        rc = fail("autotune: %s", hipGetErrorString(e));
+        h->L.chunk = keep.chunk;
+        h->L.order = keep.order;
+    } else {
+        h->L.chunk = cand[best].chunk;
+        h->L.order = cand[best].order;
+    }
+    // restore the forces of the state and the items of the chosen decomposition
+    (void)hipMemcpyAsync(h->acc, acc_save, N * sizeof(float4), hipMemcpyDeviceToDevice, h->stream);
+    (void)hipMemcpyAsync(h->dthdt, dth_save, N * sizeof(float), hipMemcpyDeviceToDevice, h->stream);
+    (void)hipMemcpyAsync(h->maxv, maxv_save, nmv * sizeof(uint32_t), hipMemcpyDeviceToDevice, h->stream);
+    launch_brick_items(h->L, h->stream, h->cstart);
+    (void)hipStreamSynchronize(h->stream);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(acc_save);
+    (void)hipFree(dth_save);
+    (void)hipFree(maxv_save);
+    if (rep) {
+        rep->n = (int32_t)std::min<size_t>(cand.size(), ASPH_TUNE_MAX);
+        for (int32_t k = 0; k < rep->n; ++k) {
+            rep->cand[k] = cand[k];
+            rep->ms[k] = ms[k];
+        }
+        rep->best = (int32_t)best;
+    }
+    if (rc == ASPH_OK) LAUNCHCHK();
+    return rc;
 }
 
 int asph_set_counters(asph_t* h, int32_t enable) {

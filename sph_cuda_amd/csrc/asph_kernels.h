@@ -19,7 +19,9 @@ struct Launch {
     uint32_t* nitems;        // number of valid items
     uint32_t* item_hist;     // [2 * brick_item_bins()] counting-sort scratch (zero between builds)
     uint32_t* item_meta;     // [nbblk] scratch
-    int nitem_max;           // grid of the brick kernels
+    int nitem_max;           // grid of the brick kernels (bound for the smallest chunk)
+    int chunk;               // listed particles per work item, at most (tuning)
+    int order;               // 1: items heaviest first, 0: brick (key) order (tuning)
     // runs of empty keys longer than cell_gap_serial(), filled by whole workgroups
     uint4* gaps;             // [gapcap]: (first key, last key, value)
     uint32_t* ngaps;         // reset by the keys kernel of each sort
@@ -75,6 +77,8 @@ void launch_momentum_kdk(const Launch& L, int S, hipStream_t s, const MomArgs& A
 int brick_bits();
 int brick_item_bins();
 int brick_item_capacity(int nbblk, int64_t n);
+int brick_chunk_min();
+int brick_chunk_max();
 void launch_brick_items(const Launch& L, hipStream_t s, const uint32_t* cstart);
 void launch_rain(const Launch& L, hipStream_t s, const int64_t* rain_in, const int64_t* dep,
                  int64_t* rain_out, int64_t* precip, unsigned long long* clipped, float dt);

@@ -45,7 +45,8 @@ constexpr int BS = 8;              // brick edge in cells
 constexpr int BT = 512;            // threads per workgroup (one per brick cell)
 constexpr int BW = BT / 64;        // wavefronts per workgroup
 constexpr int QK = 8;              // queued in-support pairs per thread
-constexpr int CHUNK = 768;         // listed particles per work item (at most)
+constexpr int CHUNK_MIN = 256;     // bounds of the work-item size (tuned at run time)
+constexpr int CHUNK_MAX = 1024;
 constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cells]
 
 __device__ __forceinline__ void min_image(const DevParams& P, float& rx, float& ry) {
@@ -637,11 +638,11 @@ __device__ __forceinline__ bool member_mode_a(const DevParams& P, const Smem<S, 
 }
 
 // Owned range of work item `it` (brick it.x, chunk it.y): a brick's listed particles are split
-// into equal chunks of at most CHUNK.
+// into equal chunks of at most `csize`.
 __device__ __forceinline__ void item_range(const uint32_t* __restrict__ cstart, uint32_t k0, uint32_t kb, int chunk,
-                                           uint32_t& o0, uint32_t& o1) {
+                                           uint32_t csize, uint32_t& o0, uint32_t& o1) {
     const uint32_t b0 = cstart[k0 + kb], b1 = cstart[k0 + kb + (1u << NB)];
-    const uint32_t n = b1 - b0, ni = (n + CHUNK - 1) / CHUNK;
+    const uint32_t n = b1 - b0, ni = (n + csize - 1) / csize;
     const uint32_t q = ni ? (n + ni - 1) / ni : 0u;
     o0 = b0 + (uint32_t)chunk * q;
     o1 = min(b1, o0 + q);
@@ -652,7 +653,7 @@ __device__ __forceinline__ void item_range(const uint32_t* __restrict__ cstart, 
 template <int S>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_density_eos_brick(
         const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
-        const uint32_t* __restrict__ nitems, int nbblk, int n_members, DensArgs A) {
+        const uint32_t* __restrict__ nitems, int nbblk, int n_members, int csize, DensArgs A) {
     __shared__ Smem<S, false> sm;
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
@@ -691,7 +692,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     sfc_params(P, A.clock ? A.clock[0] : A.simt, K.Ts, K.qs_sfc);
     const uint32_t kb = (uint32_t)bm.base;
     uint32_t o0, o1;
-    item_range(A.cstart, k0, kb, it.y, o0, o1);
+    item_range(A.cstart, k0, kb, it.y, (uint32_t)csize, o0, o1);
     if (o0 >= o1) return;
     const BrickGeo g = brick_geo<S, false>(P, sm, kb);
     plan<S, false>(P, k0, g, A.cstart, sm);
@@ -807,7 +808,7 @@ __device__ __forceinline__ void reduce_maxv(uint32_t* maxv, float mx, float my, 
 template <int S>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_momentum_kdk_brick(
         const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
-        const uint32_t* __restrict__ nitems, int nbblk, MomArgs A) {
+        const uint32_t* __restrict__ nitems, int nbblk, int csize, MomArgs A) {
     __shared__ Smem<S, true> sm;
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
@@ -840,7 +841,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const uint32_t k0 = MD.k0;
     const uint32_t kb = (uint32_t)bm.base;
     uint32_t o0, o1;
-    item_range(A.cstart, k0, kb, it.y, o0, o1);
+    item_range(A.cstart, k0, kb, it.y, (uint32_t)csize, o0, o1);
     if (o0 >= o1) return;
     float mx = 0.f, my = 0.f, mz = 0.f;
     const BrickGeo g = brick_geo<S, true>(P, sm, kb);
@@ -867,10 +868,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 }
 
 // ---------------------------------------------------------------------------------------------
-// Work items of the brick kernels: every non-empty brick is split into chunks of at most CHUNK
-// listed particles, and the chunks are ordered by the brick's estimated cost (chunk size x brick
-// occupancy, 4 bins per octave), heaviest first, so that the long workgroups of dense regions
-// start early instead of forming the tail of the launch. Counting sort: count, scan, fill.
+// Work items of the brick kernels: every non-empty brick is split into chunks of at most `csize`
+// listed particles, and (order = 1) the chunks are ordered by the brick's estimated cost (chunk
+// size x brick occupancy, 4 bins per octave), heaviest first, so that the long workgroups of dense
+// regions start early instead of forming the tail of the launch. Counting sort: count, scan, fill.
 // ---------------------------------------------------------------------------------------------
 constexpr int ITEM_BINS = 128;
 
@@ -878,6 +879,7 @@ constexpr int ITEM_BINS = 128;
 // atomic per bin and workgroup: most bricks share a bin).
 __global__ __launch_bounds__(256) void k_items_count(const MemberDesc* __restrict__ mem,
                                                      const BlockMap* __restrict__ bmap, int nbblk,
+                                                     uint32_t csize, int order,
                                                      const uint32_t* __restrict__ cstart,
                                                      uint32_t* __restrict__ meta, uint32_t* __restrict__ hist) {
     __shared__ uint32_t lh[ITEM_BINS];
@@ -889,12 +891,14 @@ __global__ __launch_bounds__(256) void k_items_count(const MemberDesc* __restric
         const BlockMap bm = bmap[b];
         const uint32_t k0 = mem[bm.m].k0, kb = (uint32_t)bm.base;
         const uint32_t n = cstart[k0 + kb + (1u << NB)] - cstart[k0 + kb];
-        const uint32_t ni = (n + CHUNK - 1) / CHUNK;
+        const uint32_t ni = (n + csize - 1) / csize;
         int bin = 0;
         if (ni) {
-            const float cost = (float)min(n, (uint32_t)CHUNK) * (float)n;
-            const int lg = (int)(4.0f * log2f(fmaxf(cost, 1.0f)));
-            bin = ITEM_BINS - 1 - min(lg, ITEM_BINS - 1);
+            if (order) {
+                const float cost = (float)min(n, csize) * (float)n;
+                const int lg = (int)(4.0f * log2f(fmaxf(cost, 1.0f)));
+                bin = ITEM_BINS - 1 - min(lg, ITEM_BINS - 1);
+            }
             atomicAdd(&lh[bin], ni);
         }
         meta[b] = (ni << 8) | (uint32_t)bin;
@@ -948,11 +952,14 @@ __global__ __launch_bounds__(256) void k_items_fill(int nbblk, const uint32_t* _
 
 int brick_bits() { return NB; }
 int brick_item_bins() { return ITEM_BINS; }
-int brick_item_capacity(int nbblk, int64_t n) { return nbblk + (int)((n + CHUNK - 1) / CHUNK); }
+int brick_item_capacity(int nbblk, int64_t n) { return nbblk + (int)((n + CHUNK_MIN - 1) / CHUNK_MIN); }
+int brick_chunk_min() { return CHUNK_MIN; }
+int brick_chunk_max() { return CHUNK_MAX; }
 
 void launch_brick_items(const Launch& L, hipStream_t s, const uint32_t* cstart) {
     const int g = (L.nbblk + 255) / 256;
-    k_items_count<<<g, 256, 0, s>>>(L.mem, L.bmap, L.nbblk, cstart, L.item_meta, L.item_hist);
+    k_items_count<<<g, 256, 0, s>>>(L.mem, L.bmap, L.nbblk, (uint32_t)L.chunk, L.order, cstart, L.item_meta,
+                                    L.item_hist);
     k_items_scan<<<1, ITEM_BINS, 0, s>>>(L.item_hist, L.nitems);
     k_items_fill<<<g, 256, 0, s>>>(L.nbblk, L.item_meta, L.item_hist, L.items);
 }
@@ -960,20 +967,20 @@ void launch_brick_items(const Launch& L, hipStream_t s, const uint32_t* cstart) 
 void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A) {
     dim3 g(L.nitem_max);
     switch (S) {
-        case 1: k_density_eos_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, A); break;
-        case 2: k_density_eos_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, A); break;
-        case 3: k_density_eos_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, A); break;
-        default: k_density_eos_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, A); break;
+        case 1: k_density_eos_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, L.chunk, A); break;
+        case 2: k_density_eos_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, L.chunk, A); break;
+        case 3: k_density_eos_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, L.chunk, A); break;
+        default: k_density_eos_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, L.chunk, A); break;
     }
 }
 
 void launch_momentum_kdk(const Launch& L, int S, hipStream_t s, const MomArgs& A) {
     dim3 g(L.nitem_max);
     switch (S) {
-        case 1: k_momentum_kdk_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, A); break;
-        case 2: k_momentum_kdk_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, A); break;
-        case 3: k_momentum_kdk_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, A); break;
-        default: k_momentum_kdk_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, A); break;
+        case 1: k_momentum_kdk_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, A); break;
+        case 2: k_momentum_kdk_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, A); break;
+        case 3: k_momentum_kdk_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, A); break;
+        default: k_momentum_kdk_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, A); break;
     }
 }
 

@@ -274,6 +274,24 @@ class Sim:
                 "asph_kernel_times")
         return {L.KERNEL_NAMES[k]: (float(ms[k]), int(cnt[k])) for k in range(L.K_COUNT)}
 
+    def tuning(self):
+        t = L.Tuning()
+        L.check(self.lib.asph_get_tuning(self.h, C.byref(t)), "asph_get_tuning")
+        return {"chunk": t.chunk, "order": t.order}
+
+    def set_tuning(self, chunk: int, order: int):
+        t = L.Tuning(chunk, order)
+        L.check(self.lib.asph_set_tuning(self.h, C.byref(t)), "asph_set_tuning")
+
+    def autotune(self, reps: int = 3):
+        """Times every brick-decomposition shape on the current state and keeps the fastest.
+        Returns {"best": {...}, "trials": [(chunk, order, ms), ...]}."""
+        r = L.TuneReport()
+        L.check(self.lib.asph_autotune(self.h, reps, C.byref(r)), "asph_autotune")
+        trials = [(r.cand[k].chunk, r.cand[k].order, r.ms[k]) for k in range(r.n)]
+        return {"best": {"chunk": r.cand[r.best].chunk, "order": r.cand[r.best].order},
+                "trials": trials}
+
     def set_counters(self, on: bool):
         L.check(self.lib.asph_set_counters(self.h, 1 if on else 0), "asph_set_counters")
 

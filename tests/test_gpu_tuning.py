@@ -1,0 +1,79 @@
+"""GPU: the brick decomposition's shape (work-item size, launch order) and the autotuner that
+picks it change which workgroup computes a particle, never the result."""
+import numpy as np
+import pytest
+
+import sph_cuda_amd as s
+from helpers import small_config
+
+pytestmark = pytest.mark.gpu
+
+FIELDS = ("pos", "vel", "thq", "rho")
+
+
+def _state(kind, compress=None):
+    cfg, ic, dt = small_config("moist", (32, 32, 24))
+    st = s.ic_generate(cfg, ic)
+    if compress:
+        st.pos[:, :3] = (st.pos[:, :3] / np.asarray(compress, np.float32)).astype(np.float32)
+    return cfg, dt, st
+
+
+def _run(cfg, dt, st, tuning=None, steps=4):
+    sim = s.Sim(cfg)
+    sim.set_state(st)
+    sim.sort()
+    if tuning:
+        sim.set_tuning(*tuning)
+    sim.force(dt)
+    for k in range(steps):
+        sim.step(dt, k * dt, do_sort=(k % 2 == 1))
+    return sim
+
+
+@pytest.mark.parametrize("compress", [None, (1.7, 1.7, 1.7)])
+@pytest.mark.parametrize("tuning", [(256, 1), (384, 0), (512, 1), (1024, 0)])
+def test_shape_does_not_change_results(compress, tuning):
+    cfg, dt, st = _state("moist", compress)
+    print(f"[case] compress={compress} tuning={tuning}", flush=True)
+    ref = _run(cfg, dt, st).get_state()
+    got = _run(cfg, dt, st, tuning).get_state()
+    for f in FIELDS:
+        assert np.array_equal(getattr(ref, f), getattr(got, f)), f
+
+
+def test_autotune_keeps_state_and_steps():
+    cfg, dt, st = _state("moist", (1.2, 1.2, 1.2))
+    a, b = s.Sim(cfg), s.Sim(cfg)
+    for x in (a, b):
+        x.set_state(st)
+        x.sort()
+        x.force(dt)
+        x.step(dt, 0.0, do_sort=True)
+    before = b.get_state()
+    acc0, prs0, dth0 = b.get_aux()
+    vmax0 = b.max_vel()
+    rep = b.autotune(reps=2)
+    print(f"[case] autotune {rep}", flush=True)
+    assert len(rep["trials"]) == 10 and all(t[2] > 0 for t in rep["trials"])
+    assert b.tuning() == rep["best"]
+    after = b.get_state()
+    acc1, prs1, dth1 = b.get_aux()
+    for f in FIELDS:
+        assert np.array_equal(getattr(before, f), getattr(after, f)), f
+    assert np.array_equal(acc0, acc1) and np.array_equal(dth0, dth1) and np.array_equal(prs0, prs1)
+    assert np.array_equal(vmax0, b.max_vel())
+    for k in range(3):
+        for x in (a, b):
+            x.step(dt, (k + 1) * dt, do_sort=(k == 1))
+    sa, sb = a.get_state(), b.get_state()
+    for f in FIELDS:
+        assert np.array_equal(getattr(sa, f), getattr(sb, f)), f
+
+
+def test_tuning_bounds_rejected():
+    cfg, dt, st = _state("moist")
+    sim = s.Sim(cfg)
+    for bad in ((128, 1), (2048, 0), (512, 2)):
+        with pytest.raises(s.AsphError):
+            sim.set_tuning(*bad)
