@@ -280,6 +280,37 @@ int asph_column_diag(asph_t* h, float* out4);
 int asph_write_dump(asph_t* h, const char* outdir, int32_t sx, int32_t sy, int32_t sz,
                     double simt, float dt, char* path_out, size_t path_cap);
 
+/* --- adjoint ------------------------------------------------------------------------------- */
+
+/* Adjoint of the rain mass with respect to the initial state and the autoconversion threshold
+ * (no reference counterpart). The forward window starts from the handle's state as set by
+ * asph_set_state (sort, force evaluation asph_h100.cu:913-920), runs nsteps KDK steps of the
+ * reference loop (asph_h100.cu:962-1038) at fixed dt, and rebuilds the neighbour structure at the
+ * start of every segment of `checkpoint` steps (the reference's stale cell lists in between). The
+ * objective J is the rain water in the domain at the end of the window [kg] (the sum over columns
+ * of the column-integrated rain). The reverse sweep recomputes each segment from its checkpoint
+ * and differentiates every stage: density, EOS + microphysics, momentum, kick/drift, sedimentation.
+ * FP32 state, FP64 reductions (rain grid, threshold derivative). */
+typedef struct asph_adjoint_args {
+    int64_t nsteps;       /* forward window [steps] */
+    float dt;             /* fixed step [s] */
+    int32_t checkpoint;   /* steps per segment (0: 10) */
+} asph_adjoint_args;
+
+typedef struct asph_adjoint_result {
+    double J;             /* rain water in the domain at the end of the window [kg] */
+    double dJ_dqc0;       /* dJ / d(autoconversion threshold qc_auto) [kg per kg/kg] */
+    double forward_s;     /* wall time of the forward window (with checkpoint copies) */
+    double adjoint_s;     /* wall time of the reverse sweep (with the recomputation) */
+    int64_t segments;
+} asph_adjoint_result;
+
+/* Single-domain handles with nx, ny >= 2*stencil+3. dJ_dtheta0 [N] (or NULL) receives dJ/dtheta of
+ * the initial state, dJ_dthq4 [4N] (or NULL) dJ/d(theta, qv, qc, -), both in the particle order of
+ * the state given to asph_set_state. The handle is left holding that initial state. */
+int asph_adjoint_rain(asph_t* h, const asph_adjoint_args* a, float* dJ_dtheta0, float* dJ_dthq4,
+                      asph_adjoint_result* r);
+
 /* --- tuning of the neighbour kernels ----------------------------------------------------- */
 
 /* Shape of the brick decomposition (no reference counterpart). Each 8x8x8-cell brick's listed

@@ -1,0 +1,393 @@
+// asph_adjoint_ref.cpp - FP64 host adjoint of the rain objective (asph_adjoint_rain).
+//
+// TEST INFRASTRUCTURE ONLY: the reference the GPU adjoint is checked against, beside central
+// finite differences of the FP64 oracle. It is loaded by tests/ and bench.py's checks only.
+//
+// The forward is the FP64 instance of the oracle (asph_oracle.hpp), recorded step by step in host
+// memory (no checkpoints). The reverse sweep differentiates the same stages with the same
+// pointwise and pair derivative formulas as the device (asph_adjoint.h, instantiated for double)
+// but with its own orchestration: plain loops over the oracle's row-major cell ranges, the
+// stencil tests written out, a full tape instead of recomputation, and double everywhere. It
+// pins the device's gather and bookkeeping and measures the FP32 effects; the finite
+// differences pin the formulas.
+#include <algorithm>
+using std::max;
+using std::min;
+
+#include "asph_adjoint.h"
+#include "asph_oracle.hpp"
+
+namespace {
+
+using oracle::Oracle;
+using oracle::Params;
+
+// The device library's derived parameters (float evaluation, as make_params).
+asph::DevParams devparams(const asph_config& c) {
+    asph::DevParams D;
+    std::memset(&D, 0, sizeof(D));
+    Params<float> P(c);
+    D.nx = c.nx; D.ny = c.ny; D.nz = c.nz; D.S = P.S;
+    D.N = c.nx * c.ny * c.nz; D.ncell = D.N; D.ncol = c.nx * c.ny;
+    D.end_bit = oracle::sort_end_bit(c.nx, c.ny, c.nz);
+    D.flags = c.flags;
+    D.dx = P.dx; D.dy = P.dy; D.dz = P.dz; D.inv_dx = P.inv_dx; D.inv_dy = P.inv_dy; D.inv_dz = P.inv_dz;
+    D.Lx = P.Lx; D.Ly = P.Ly; D.Lz = P.Lz; D.hLx = P.hLx; D.hLy = P.hLy;
+    D.inv_h = P.inv_h; D.lim = P.lim; D.kin = P.kin; D.kout = P.kout;
+    D.vcell = P.vcell; D.rho0 = P.rho0; D.pmass = P.pmass; D.mm = P.mm; D.eta2 = P.eta2; D.visc_c = P.visc_c;
+    D.cs = P.cs; D.cs2 = P.cs2; D.th0 = P.th0; D.n2 = P.n2; D.exA = P.exA; D.cp_rd = P.cp_rd; D.rd_cp = P.rd_cp;
+    D.sponge_sigma = P.sponge_sigma; D.sponge_frac = P.sponge_frac;
+    D.sponge_zs = (1.0f - c.sponge_frac) * P.Lz; D.sponge_depth = c.sponge_frac * P.Lz;
+    D.dthdt_cap = P.dthdt_cap; D.c_h = P.c_h; D.c_e = P.c_e; D.beta_moist = P.beta_moist; D.v_bg = P.v_bg;
+    D.dts_skin = P.dts_skin; D.ramp = P.ramp;
+    D.qc_auto = P.qc_auto; D.rate_auto = P.rate_auto; D.k_accr = P.k_accr; D.v_rain = P.v_rain;
+    D.theta_cap = P.theta_cap; D.qv_cap = P.qv_cap;
+    D.qv_mode = P.qv_mode; D.qv0 = P.qv0; D.qv_hs = P.qv_hs; D.qv_zt = P.qv_zt; D.qv_ht = P.qv_ht;
+    D.rh_n = P.rh_n;
+    for (int k = 0; k < ASPH_RH_KNOTS; ++k) { D.rh_z[k] = c.rh_z[k]; D.rh_v[k] = c.rh_v[k]; }
+    return D;
+}
+
+struct Tape {
+    std::vector<double> x0, v0, a0, t0, r0, x1, v1, r1, s12, t1, prs, a1, av;
+    std::vector<int64_t> rain0;
+    std::vector<uint32_t> keys;
+    std::vector<int32_t> cell_range;
+    double Ts = 0.0, qs = 0.0;   // surface skin temperature and saturation of the step
+};
+
+struct Adj {
+    std::vector<double> x, v, a, t, rho, rain;
+    explicit Adj(int64_t N) : x(4 * N), v(4 * N), a(4 * N), t(4 * N), rho(N), rain(N) {}
+};
+
+void rec_pre(const Oracle<double>& o, Tape& T) {
+    T.x0 = o.pos; T.v0 = o.vel; T.a0 = o.acc; T.t0 = o.thq; T.r0 = o.rho; T.rain0 = o.rain;
+    T.keys = o.keys_sorted; T.cell_range = o.cell_range;
+}
+
+void rec_mid(const Oracle<double>& o, Tape& T) {
+    T.x1 = o.pos; T.v1 = o.vel; T.r1 = o.rho; T.t1 = o.thq; T.prs = o.prs;
+}
+
+int wrapd(int d, int n) {   // periodic offset in (-n/2, n/2]
+    d %= n;
+    if (d < 0) d += n;
+    if (d > n / 2) d -= n;
+    return d;
+}
+
+bool in_stencil(const asph::DevParams& P, int ix, int iy, int iz, int cx, int cy, int cz) {
+    const int S = P.S;
+    const bool interior = (iz >= S && iz < P.nz - S);
+    const int z0 = interior ? iz - S : std::max(iz - S, 0);
+    const int z1 = interior ? iz + S : std::min(iz + S, P.nz - 1);
+    if (cz < z0 || cz > z1) return false;
+    return std::abs(wrapd(cx - ix, P.nx)) <= S && std::abs(wrapd(cy - iy, P.ny)) <= S;
+}
+
+void cur_cell(const Params<double>& Q, const double* p, int& ix, int& iy, int& iz) { Q.cell_trunc(p, ix, iy, iz); }
+
+void sep(const asph::DevParams& P, const double* a, const double* b, double r[3]) {
+    r[0] = a[0] - b[0]; r[1] = a[1] - b[1]; r[2] = a[2] - b[2];
+    if (P.flags & ASPH_F_MIN_IMAGE) {
+        const double Lx = P.Lx, Ly = P.Ly;
+        if (r[0] > 0.5 * Lx) r[0] -= Lx; else if (r[0] < -0.5 * Lx) r[0] += Lx;
+        if (r[1] > 0.5 * Ly) r[1] -= Ly; else if (r[1] < -0.5 * Ly) r[1] += Ly;
+    }
+}
+
+// Calls fn(j, own, rev) for the particles j listed within S + 1 cells of i's listed cell for which
+// (i, j) is a pair of i's sum (own) or (j, i) one of j's sum (rev), by the cell tests of the forward.
+template <class Fn>
+void for_pairs(const asph::DevParams& P, const Params<double>& Q, const Tape& T, int64_t i, Fn fn) {
+    const uint32_t key = T.keys[i];
+    const int lx = (int)asph::compact1by2(key), ly = (int)asph::compact1by2(key >> 1),
+              lz = (int)asph::compact1by2(key >> 2);
+    int ix, iy, iz;
+    cur_cell(Q, &T.x1[4 * i], ix, iy, iz);
+    const int M = P.S + 1;
+    for (int cz = std::max(lz - M, 0); cz <= std::min(lz + M, P.nz - 1); ++cz)
+        for (int dy = -M; dy <= M; ++dy)
+            for (int dx = -M; dx <= M; ++dx) {
+                const int cx = Params<double>::wrap(lx + dx, P.nx), cy = Params<double>::wrap(ly + dy, P.ny);
+                const int64_t c = ((int64_t)cz * P.ny + cy) * P.nx + cx;
+                const bool own = in_stencil(P, ix, iy, iz, cx, cy, cz);
+                for (int32_t j = T.cell_range[2 * c]; j < T.cell_range[2 * c + 1]; ++j) {
+                    int jx, jy, jz;
+                    cur_cell(Q, &T.x1[4 * (int64_t)j], jx, jy, jz);
+                    const bool rev = in_stencil(P, jx, jy, jz, lx, ly, lz);
+                    if (own || rev) fn((int64_t)j, own, rev);
+                }
+            }
+}
+
+// Reverse of one step (mode 1) or of the force evaluation (mode 0): E (adjoints after) -> S.
+void reverse(const asph::DevParams& P, const Params<double>& Q, const Tape& T, const Adj& E, Adj& S, double dt,
+             int mode, double& qc0b) {
+    const int64_t N = P.N, plane = (int64_t)P.nx * P.ny;
+    const bool sed = mode == 1 && (P.flags & ASPH_F_SEDIMENT);
+    const double frac = sed ? (double)asph::rain_frac(P, (float)dt) : 0.0;
+    std::vector<double> rb_cur(N);
+    for (int64_t c = 0; c < N; ++c)
+        rb_cur[c] = sed ? (1.0 - frac) * E.rain[c] + (c >= plane ? frac * E.rain[c - plane] : 0.0) : E.rain[c];
+    std::vector<double> Ab(4 * N), AVb(4 * N), x1b(4 * N), v1b(4 * N), t2b(4 * N), pb(N), r1b(N), sb(2 * N);
+    const double mm = P.mm;
+    // momentum epilogue
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < N; ++i) {
+        const double* x1 = &T.x1[4 * i];
+        const double* t1 = &T.t1[4 * i];
+        double in[asph::adj::MOM_IN], ob[asph::adj::MOM_OUT], ib[asph::adj::MOM_IN];
+        const double thv = t1[0] * (1.0 + 0.608 * t1[1]);
+        const double thvr = Q.theta_ref(x1[2]) * (1.0 + 0.608 * Q.qv_ref(x1[2]));
+        const double buoy = Q.GRAV * ((thv - thvr) / thvr - t1[2]);
+        in[0] = T.a1[4 * i]; in[1] = T.a1[4 * i + 1]; in[2] = T.a1[4 * i + 2] - buoy;
+        for (int a = 0; a < 3; ++a) { in[3 + a] = T.av[4 * i + a]; in[6 + a] = T.v1[4 * i + a]; }
+        in[9] = x1[2]; in[10] = t1[0]; in[11] = t1[1]; in[12] = t1[2]; in[13] = T.prs[i];
+        ob[0] = E.a[4 * i]; ob[1] = E.a[4 * i + 1]; ob[2] = E.a[4 * i + 2]; ob[3] = E.t[4 * i];
+        ob[4] = E.v[4 * i]; ob[5] = E.v[4 * i + 1]; ob[6] = E.v[4 * i + 2];
+        asph::adj::mom_vjp<double>(P, dt, mode, in, ob, ib);
+        for (int a = 0; a < 3; ++a) {
+            Ab[4 * i + a] = ib[a] * mm;
+            AVb[4 * i + a] = ib[3 + a] * mm;
+            x1b[4 * i + a] = E.x[4 * i + a];
+            v1b[4 * i + a] = ib[6 + a];
+        }
+        x1b[4 * i + 2] += ib[9];
+        t2b[4 * i] = ib[10];
+        t2b[4 * i + 1] = E.t[4 * i + 1] + ib[11];
+        t2b[4 * i + 2] = E.t[4 * i + 2] + ib[12];
+        pb[i] = ib[13];
+        r1b[i] = E.rho[i];
+    }
+    // momentum pairs
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < N; ++i) {
+        const double* xi = &T.x1[4 * i];
+        const double vi[3] = {T.v1[4 * i], T.v1[4 * i + 1], T.v1[4 * i + 2]};
+        const double rho_i = std::max(T.r1[i], 1e-3), p_i = T.prs[i];
+        double xb[3] = {0, 0, 0}, vb[3] = {0, 0, 0}, rb = 0, pbi = 0;
+        for_pairs(P, Q, T, i, [&](int64_t j, bool own, bool rev) {
+            const double* xj = &T.x1[4 * j];
+            const double vj[3] = {T.v1[4 * j], T.v1[4 * j + 1], T.v1[4 * j + 2]};
+            const double rho_j = std::max(T.r1[j], 1e-3), p_j = T.prs[j];
+            asph::adj::PairBar<double> b;
+            double r[3];
+            if (own) {
+                sep(P, xi, xj, r);
+                if (asph::adj::mom_pair_vjp<double>(P, r[0], r[1], r[2], vi, vj, rho_i, rho_j, p_i, p_j,
+                                                    &Ab[4 * i], &AVb[4 * i], b)) {
+                    for (int a = 0; a < 3; ++a) { xb[a] += b.r[a]; vb[a] += b.vd[a]; }
+                    rb += b.rho_o;
+                    pbi += b.p_o;
+                }
+            }
+            if (rev) {
+                sep(P, xj, xi, r);
+                if (asph::adj::mom_pair_vjp<double>(P, r[0], r[1], r[2], vj, vi, rho_j, rho_i, p_j, p_i,
+                                                    &Ab[4 * j], &AVb[4 * j], b)) {
+                    for (int a = 0; a < 3; ++a) { xb[a] -= b.r[a]; vb[a] -= b.vd[a]; }
+                    rb += b.rho_n;
+                    pbi += b.p_n;
+                }
+            }
+        });
+        for (int a = 0; a < 3; ++a) { x1b[4 * i + a] += xb[a]; v1b[4 * i + a] += vb[a]; }
+        if (T.r1[i] > 1e-3) r1b[i] += rb;
+        pb[i] += pbi;
+    }
+    // EOS + microphysics
+    S.rain = rb_cur;
+    const double Ts = T.Ts, qs = T.qs;
+    for (int64_t i = 0; i < N; ++i) {
+        const double* x1 = &T.x1[4 * i];
+        const int64_t cid = Q.cell_of(x1);
+        asph::adj::EosConst<double> K;
+        K.dt = dt;
+        K.auto_f = 1.0 - std::exp(-Q.rate_auto * dt);   // as the FP64 forward (no float cancellation)
+        K.Ts = Ts;
+        K.qs_sfc = qs;
+        K.flux = mode == 1 && (P.flags & ASPH_F_SURFACE_FLUX) && (T.keys[i] & 0x24924924u) == 0u && x1[2] < (double)P.dz;
+        const double in[asph::adj::EOS_IN] = {T.t0[4 * i], T.t0[4 * i + 1], T.t0[4 * i + 2], T.r1[i], x1[2],
+                                              (double)T.rain0[cid] / Oracle<double>::FXS, (double)P.qc_auto};
+        const double ob[asph::adj::EOS_OUT] = {t2b[4 * i], t2b[4 * i + 1], t2b[4 * i + 2], pb[i], rb_cur[cid]};
+        double ib[asph::adj::EOS_IN];
+        asph::adj::eos_vjp<double>(P, K, in, ob, ib);
+        S.t[4 * i] = ib[0]; S.t[4 * i + 1] = ib[1]; S.t[4 * i + 2] = ib[2]; S.t[4 * i + 3] = 0;
+        r1b[i] += ib[3];
+        x1b[4 * i + 2] += ib[4];
+        S.rain[cid] += ib[5];
+        qc0b += ib[6];
+    }
+    // density ratio
+    for (int64_t i = 0; i < N; ++i) {
+        double zb;
+        asph::adj::dens_vjp<double>(P, T.s12[2 * i], T.s12[2 * i + 1], T.x1[4 * i + 2], r1b[i], sb[2 * i],
+                                    sb[2 * i + 1], zb);
+        x1b[4 * i + 2] += zb;
+    }
+    // density pairs
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t i = 0; i < N; ++i) {
+        const double* xi = &T.x1[4 * i];
+        const double inv_r0i = 1.0 / std::max(T.r0[i], 1e-3);
+        double xb[3] = {0, 0, 0}, rb = 0;
+        for_pairs(P, Q, T, i, [&](int64_t j, bool own, bool rev) {
+            const double* xj = &T.x1[4 * j];
+            double r[3], W, gw[3];
+            if (own) {
+                sep(P, xi, xj, r);
+                if (asph::adj::dens_pair<double>(P, r[0], r[1], r[2], W, gw)) {
+                    const double wb = sb[2 * i] + sb[2 * i + 1] * (1.0 / std::max(T.r0[j], 1e-3));
+                    for (int a = 0; a < 3; ++a) xb[a] += wb * gw[a];
+                }
+            }
+            if (rev) {
+                sep(P, xj, xi, r);
+                if (asph::adj::dens_pair<double>(P, r[0], r[1], r[2], W, gw)) {
+                    const double wb = sb[2 * j] + sb[2 * j + 1] * inv_r0i;
+                    for (int a = 0; a < 3; ++a) xb[a] -= wb * gw[a];
+                    rb += sb[2 * j + 1] * W * (-inv_r0i * inv_r0i);
+                }
+            }
+        });
+        for (int a = 0; a < 3; ++a) x1b[4 * i + a] += xb[a];
+        S.rho[i] = T.r0[i] > 1e-3 ? rb : 0.0;
+    }
+    // kick + drift
+    for (int64_t i = 0; i < N; ++i) {
+        double* sx = &S.x[4 * i];
+        double* sv = &S.v[4 * i];
+        double* sa = &S.a[4 * i];
+        const double* xb = &x1b[4 * i];
+        const double* vb = &v1b[4 * i];
+        if (mode == 0) {
+            for (int a = 0; a < 3; ++a) { sx[a] = xb[a]; sv[a] = vb[a]; sa[a] = 0; }
+            sx[3] = sv[3] = sa[3] = 0;
+            continue;
+        }
+        const double hdt = 0.5 * dt;
+        const double vz = std::fma(T.a0[4 * i + 2], hdt, T.v0[4 * i + 2]);
+        const double z1 = std::fma(vz, dt, T.x0[4 * i + 2]);
+        const double sg = (z1 < 0.0 || z1 > (double)Q.Lz) ? -1.0 : 1.0;
+        const double zb = sg * xb[2];
+        const double vp[3] = {vb[0] + dt * xb[0], vb[1] + dt * xb[1], sg * vb[2] + dt * zb};
+        sx[0] = xb[0]; sx[1] = xb[1]; sx[2] = zb; sx[3] = 0;
+        for (int a = 0; a < 3; ++a) { sv[a] = vp[a]; sa[a] = hdt * vp[a]; }
+        sv[3] = sa[3] = 0;
+    }
+}
+
+void unperm(const std::vector<uint32_t>& perm, Adj& A) {
+    const int64_t N = (int64_t)perm.size();
+    Adj B(N);
+    for (int64_t i = 0; i < N; ++i) {
+        const int64_t k = perm[i];
+        for (int a = 0; a < 4; ++a) {
+            B.x[4 * k + a] = A.x[4 * i + a];
+            B.v[4 * k + a] = A.v[4 * i + a];
+            B.a[4 * k + a] = A.a[4 * i + a];
+            B.t[4 * k + a] = A.t[4 * i + a];
+        }
+        B.rho[k] = A.rho[i];
+    }
+    B.rain = A.rain;
+    A = std::move(B);
+}
+
+}  // namespace
+
+extern "C" {
+
+// FP64 host adjoint of asph_adjoint_rain (same schedule and objective). Inputs are the initial
+// state in the layout of asph_set_state (float64); outputs J, dJ/d(theta, qv, qc, -) per particle
+// in that order, and dJ/d(qc_auto).
+int oracle_adjoint_rain(const asph_config* cfg, const double* pos4, const double* vel4, const double* thq4,
+                        const double* rho, int64_t nsteps, float dt_f, int ck, double* J_out, double* g_thq4,
+                        double* g_qc0) {
+    if (!cfg || !pos4 || !vel4 || !thq4 || !rho || nsteps < 1 || ck < 1) return 1;
+    const asph::DevParams P = devparams(*cfg);
+    Oracle<double> o(*cfg);
+    const int64_t N = o.N;
+    std::memcpy(o.pos.data(), pos4, 4 * N * sizeof(double));
+    std::memcpy(o.vel.data(), vel4, 4 * N * sizeof(double));
+    std::memcpy(o.thq.data(), thq4, 4 * N * sizeof(double));
+    std::memcpy(o.rho.data(), rho, N * sizeof(double));
+    const double dt = (double)dt_f;
+    const Params<double>& Q = o.P;
+    std::vector<std::vector<uint32_t>> perms;
+    std::vector<Tape> tape((size_t)nsteps + 1);   // slot 0: the force evaluation
+    std::vector<double> s12(2 * N), av(4 * N);
+    // forward
+    o.sort();
+    perms.push_back(o.perm);
+    {
+        Tape& T = tape[0];
+        rec_pre(o, T);
+        std::fill(o.dep.begin(), o.dep.end(), 0);
+        o.rec_s12 = &s12;
+        o.density();
+        o.rec_s12 = nullptr;
+        o.eos_micro(dt);
+        rec_mid(o, T);
+        T.s12 = s12;
+        o.rain_merge();
+        o.rec_av = &av;
+        o.momentum();
+        o.rec_av = nullptr;
+        T.a1 = o.acc;
+        T.av = av;
+        o.initialized = true;
+    }
+    for (int64_t k = 0; k < nsteps; ++k) {
+        const double t = (double)k * dt;
+        if (k > 0 && k % ck == 0) {
+            o.sort();
+            perms.push_back(o.perm);
+        }
+        Tape& T = tape[(size_t)k + 1];
+        o.surface_flux_params(t, T.Ts, T.qs);
+        rec_pre(o, T);
+        std::fill(o.dep.begin(), o.dep.end(), 0);
+        o.kick(0.5 * dt);
+        o.drift(dt);
+        if (o.flag(ASPH_F_SURFACE_FLUX)) o.surface_flux(dt, t);
+        o.rec_s12 = &s12;
+        o.density();
+        o.rec_s12 = nullptr;
+        o.eos_micro(dt);
+        // x1 and v1 are the positions and velocities after the drift; the flux changed thq only
+        rec_mid(o, T);
+        T.s12 = s12;
+        o.rec_av = &av;
+        o.momentum();
+        o.rec_av = nullptr;
+        T.a1 = o.acc;
+        T.av = av;
+        o.theta_update(dt);
+        o.kick(0.5 * dt);
+        if (o.flag(ASPH_F_SPONGE)) o.sponge(dt);
+        o.rain_adv(dt);
+    }
+    long double sum = 0;
+    for (int64_t c = 0; c < N; ++c) sum += (long double)o.rain[c];
+    *J_out = (double)(sum / (long double)Oracle<double>::FXS) * (double)P.pmass;
+    // reverse
+    Adj E(N), S(N);
+    std::fill(E.rain.begin(), E.rain.end(), (double)P.pmass);
+    double qc0b = 0.0;
+    for (int64_t k = nsteps - 1; k >= 0; --k) {
+        reverse(P, Q, tape[(size_t)k + 1], E, S, dt, 1, qc0b);
+        std::swap(E, S);
+        if (k > 0 && k % ck == 0) unperm(perms[(size_t)(k / ck)], E);
+    }
+    reverse(P, Q, tape[0], E, S, dt, 0, qc0b);
+    std::swap(E, S);
+    unperm(perms[0], E);
+    std::memcpy(g_thq4, E.t.data(), 4 * N * sizeof(double));
+    *g_qc0 = qc0b;
+    return 0;
+}
+
+}  // extern "C"

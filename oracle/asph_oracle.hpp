@@ -25,6 +25,7 @@
 #include <cstdint>
 #include <cstring>
 #include <numeric>
+#include <type_traits>
 #include <vector>
 
 #include "asph.h"
@@ -228,6 +229,10 @@ template <class R> struct Oracle {
     double simt = 0.0, since_sort = 1e9;
     float sort_dt = 0.5f;
     bool initialized = false;
+    // tape hooks of the FP64 host adjoint (null otherwise): density sums (S1, S2) and the viscous
+    // acceleration sums times mm (x, y, z, 0) of the last evaluation
+    std::vector<R>* rec_s12 = nullptr;
+    std::vector<R>* rec_av = nullptr;
 
     explicit Oracle(const asph_config& c) : cfg(c), P(c) {
         N = P.N;
@@ -243,6 +248,12 @@ template <class R> struct Oracle {
     }
 
     bool flag(uint32_t f) const { return (P.flags & f) != 0; }
+
+    // Fixed-point unit of the rain grid: 2^-40 particle masses in the FP32 instance (as the device,
+    // for bit parity); 2^-56 in the FP64 instance, whose rain finite differences would otherwise
+    // be limited by the rounding of each deposit to a unit.
+    static constexpr double FXS = std::is_same<R, float>::value ? FX_SCALE : 72057594037927936.0;
+    static int64_t fx(double v) { return (int64_t)std::llrint(v * FXS); }
 
     // ---- neighbour structure: asph_h100.cu:229-278, 962-976 -------------------------------
     void sort() {
@@ -396,6 +407,7 @@ template <class R> struct Oracle {
                         }
                     }
                 }
+            if (rec_s12) { (*rec_s12)[2 * i] = S1; (*rec_s12)[2 * i + 1] = S2; }
             R rref = P.rho_ref(pi[2]);
             R rr = S2 > R(1e-20f) ? S1 / S2 : rref;                                    // :331
             rho_alt[i] = mmin<R>(mmax<R>(rr, R(0.5f) * rref), R(2.0f) * rref);   // :332
@@ -445,16 +457,19 @@ template <class R> struct Oracle {
             if (do_auto && qc > P.qc_auto) {                                     // :389-394
                 R dqr = (qc - P.qc_auto) * auto_f;
                 qc -= dqr;
-                dfx += to_fx((double)dqr);
+                dfx += fx((double)dqr);
             }
             if (do_acc || do_evap) {
                 cid = P.cell_of(p);
-                R rf = R((float)((double)rain[cid] / FX_SCALE));
+                // the FP32 instance reads the grid as the device does (rounded to float); the FP64
+                // instance, the finite-difference reference of the adjoint, reads it exactly
+                R rf = std::is_same<R, float>::value ? R((float)((double)rain[cid] / FXS))
+                                                     : R((double)rain[cid] / FXS);
                 R qr = (rf * P.mm) / r;
                 if (do_acc && qc > R(0) && qr > R(0)) {
                     // Kessler accretion k2 qc qr^0.875, implicit in qc
                     R qcn = qc / (R(1) + (P.k_accr * dt) * mpow<R>(qr, R(0.875f)));
-                    dfx += to_fx((double)(qc - qcn));
+                    dfx += fx((double)(qc - qcn));
                     qc = qcn;
                 }
                 if (do_evap && qr > R(0)) {
@@ -469,7 +484,7 @@ template <class R> struct Oracle {
                         R dqe = mmin<R>(mmin<R>(E * dt, qr), qs2 - qv);
                         qv += dqe;
                         th -= (P.L_V * dqe) / (P.CP * Pi);
-                        dfx -= to_fx((double)dqe);
+                        dfx -= fx((double)dqe);
                     }
                 }
             }
@@ -543,6 +558,10 @@ template <class R> struct Oracle {
                 }
             ax *= P.mm; ay *= P.mm; az *= P.mm;                                        // :497-499
             avx *= P.mm; avy *= P.mm; avz *= P.mm;
+            if (rec_av) {
+                (*rec_av)[4 * i] = avx; (*rec_av)[4 * i + 1] = avy; (*rec_av)[4 * i + 2] = avz;
+                (*rec_av)[4 * i + 3] = R(0);
+            }
             R thv = ti[0] * (R(1) + R(0.608f) * ti[1]);                                // :502-504
             R thvr = P.theta_ref(pi[2]) * (R(1) + R(0.608f) * P.qv_ref(pi[2]));
             az = az + P.GRAV * ((thv - thvr) / thvr - ti[2]);
@@ -625,7 +644,7 @@ template <class R> struct Oracle {
         double rw = 0;
         for (int64_t c = 0; c < ncell; ++c) rw += (double)(rain[c] + dep[c]);
         for (int64_t c = 0; c < ncol; ++c) rw += (double)precip[c];
-        out[6] = rw / FX_SCALE * m;
+        out[6] = rw / FXS * m;
     }
 
     // ---- dumps: asph_h100.cu:680-723 --------------------------------------------------------

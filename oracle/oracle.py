@@ -62,6 +62,8 @@ def load():
         "oracle_coarse_dump": (C.c_int, [P, C.c_int, C.c_int, C.c_int, FP]),
         "oracle_column_diag": (C.c_int, [P, FP]),
         "oracle_run": (C.c_int, [P, C.c_double, C.c_int64, C.c_float, C.c_int, DP]),
+        "oracle_adjoint_rain": (C.c_int, [C.POINTER(Config), DP, DP, DP, DP, C.c_int64, C.c_float, C.c_int,
+                                          DP, DP, DP]),
         "oracle_morton3": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
         "oracle_fexp": (C.c_float, [C.c_float]),
         "oracle_flog": (C.c_float, [C.c_float]),
@@ -247,3 +249,35 @@ class Oracle:
                                  st.ctypes.data_as(C.POINTER(C.c_double)))
         keys = ("steps", "sorts", "trips", "simt", "dt", "dt_required", "ux", "uy", "uz", "wall_s")
         return rc, dict(zip(keys, st.tolist()))
+
+
+def adjoint_rain_f64(cfg, st, nsteps, dt, checkpoint=10):
+    """FP64 host adjoint (oracle/asph_adjoint_ref.cpp) of the rain objective of asph_adjoint_rain,
+    same schedule: returns (J [kg], dJ/d(theta, qv, qc, -) of shape (N, 4), dJ/d(qc_auto))."""
+    lib = load()
+    cfg = cfg.copy()
+    arrs = [np.ascontiguousarray(a, np.float64) for a in (st.pos, st.vel, st.thq, st.rho)]
+    g = np.zeros((cfg.n, 4), np.float64)
+    J, gq = C.c_double(), C.c_double()
+    dp = C.POINTER(C.c_double)
+    rc = lib.oracle_adjoint_rain(C.byref(cfg), *[a.ctypes.data_as(dp) for a in arrs], nsteps, dt, checkpoint,
+                                 C.byref(J), g.ctypes.data_as(dp), C.byref(gq))
+    if rc:
+        raise RuntimeError("oracle_adjoint_rain failed")
+    return J.value, g, gq.value
+
+
+def rain_objective_f64(cfg, pos, vel, thq, rho, nsteps, dt, checkpoint=10):
+    """The rain objective of asph_adjoint_rain evaluated by the FP64 oracle (same schedule)."""
+    o = Oracle(cfg, fp64=True)
+    o.set_state_f64(*[np.ascontiguousarray(a, np.float64) for a in (pos, vel, thq, rho)])
+    o.sort()
+    o.force(dt)
+    for k in range(nsteps):
+        o.step(dt, k * dt, do_sort=(k > 0 and k % checkpoint == 0))
+    r, _, _ = o.get_rain_fx()
+    # particle mass as the library evaluates it: rho0 * ((dx * dy) * dz) in float
+    f = np.float32
+    rho0 = f(f(100000.0) / (f(287.05) * f(cfg.th0)))
+    pm = f(rho0 * f(f(f(cfg.dx) * f(cfg.dy)) * f(cfg.dz)))
+    return float(np.sum(r.astype(np.float64))) / 2.0**56 * float(pm)   # FP64 rain unit: 2^-56

@@ -92,6 +92,15 @@ class RunArgs(C.Structure):
     ]
 
 
+class AdjointArgs(C.Structure):
+    _fields_ = [("nsteps", C.c_int64), ("dt", C.c_float), ("checkpoint", C.c_int32)]
+
+
+class AdjointResult(C.Structure):
+    _fields_ = [("J", C.c_double), ("dJ_dqc0", C.c_double), ("forward_s", C.c_double),
+                ("adjoint_s", C.c_double), ("segments", C.c_int64)]
+
+
 class Tuning(C.Structure):
     _fields_ = [("chunk", C.c_int32), ("order", C.c_int32)]
 
@@ -162,6 +171,7 @@ SIGNATURES = {
     "asph_kernel_times": (C.c_int, [P, DP, I64P]),
     "asph_synchronize": (C.c_int, [P]),
     "asph_set_counters": (C.c_int, [P, C.c_int32]),
+    "asph_adjoint_rain": (C.c_int, [P, C.POINTER(AdjointArgs), FP, FP, C.POINTER(AdjointResult)]),
     "asph_set_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
     "asph_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
     "asph_autotune": (C.c_int, [P, C.c_int32, C.POINTER(TuneReport)]),

@@ -17,64 +17,7 @@
 
 using namespace asph;
 
-struct asph_t {
-    std::vector<asph_config> cfgs;
-    std::vector<MemberDesc> mem_h;
-    std::vector<BlockMap> pmap_h, bmap_h;
-    MemberDesc* mem_d = nullptr;
-    BlockMap* pmap_d = nullptr;
-    BlockMap* bmap_d = nullptr;
-    Launch L{};
-    int S = 0;              // stencil (common to all members)
-    int kb = 0;             // key bits of one member
-    int end_bit = 0;        // sort bits (member index above the member's key bits)
-    uint32_t nkeys = 0;
-    hipStream_t stream = nullptr;
-    bool own_stream = false;
-    int64_t N = 0, ncol = 0;
-    float4 *pos = nullptr, *vel = nullptr, *thq = nullptr, *acc = nullptr;
-    float4 *pos2 = nullptr, *vel2 = nullptr, *thq2 = nullptr, *acc2 = nullptr;
-    float *rho = nullptr, *rho2 = nullptr, *prs = nullptr, *dthdt = nullptr;
-    int64_t *rain = nullptr, *rain2 = nullptr, *dep = nullptr, *precip = nullptr;
-    unsigned long long* clipped = nullptr;
-    uint32_t *keys0 = nullptr, *kA = nullptr, *vA = nullptr, *kB = nullptr, *vB = nullptr;
-    uint32_t *bh = nullptr, *bo = nullptr, *dtot = nullptr, *cstart = nullptr;
-    uint32_t *keys_sorted = nullptr, *perm = nullptr;
-    uint32_t* maxv = nullptr;        // [n_members][3][64]
-    float* disp = nullptr;           // drift since the last sort [cells per axis]
-    uint32_t* disp_slots = nullptr;  // per-step drift maxima (64 slots per axis)
-    double* clock = nullptr;         // device simulated time (graph replay)
-    double* audit_buf = nullptr;
-    std::vector<void*> allocs;
-    bool sorted = false, initialized = false;
-    bool dep_pending = false;        // dep holds deposits not yet merged into the rain grid
-    double simt = 0.0, since_sort = 1e9;
-    float sort_dt = 0.5f;
-    // hipGraph cache: one executable per (buffer assignment, sort flag, dt); tuning changes drop it
-    std::map<std::tuple<void*, void*, void*, void*, void*, void*, int, float>, hipGraphExec_t> graphs;
-    // profiling
-    bool prof = false;
-    unsigned long long* stats = nullptr;   // work counters of the neighbour kernels (enabled)
-    unsigned long long* stats_buf = nullptr;
-    std::vector<hipEvent_t> ev_pool;
-    std::vector<std::pair<hipEvent_t, hipEvent_t>> ev_live[ASPH_K_COUNT];
-
-    int n_members() const { return (int)cfgs.size(); }
-    template <class T> int alloc(T** p, size_t count) {
-        void* q = nullptr;
-        hipError_t e = hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T));
-        if (e != hipSuccess) return fail("hipMalloc(%zu bytes): %s", count * sizeof(T), hipGetErrorString(e));
-        allocs.push_back(q);
-        *p = (T*)q;
-        return ASPH_OK;
-    }
-    hipEvent_t get_event() {
-        if (!ev_pool.empty()) { hipEvent_t e = ev_pool.back(); ev_pool.pop_back(); return e; }
-        hipEvent_t e = nullptr;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-        return e;
-    }
-};
+#include "asph_handle.h"
 
 namespace {
 
@@ -218,12 +161,18 @@ int create_impl(const asph_config* cfgs, int n, void* stream, asph_t** out) {
     return ASPH_OK;
 }
 
+}  // namespace
+
+// Step pieces shared with the adjoint driver (asph_adjoint_host.cpp).
+namespace asph_host {
+
 MomArgs mom_args(asph_t* h, float dt, int mode) {
     MomArgs A;
     A.pos = h->pos; A.vel = h->vel; A.thq = h->thq; A.rho = h->rho; A.prs = h->prs;
     A.cstart = h->cstart; A.acc = h->acc; A.dthdt = h->dthdt; A.vel_out = h->vel2;
     A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
     A.stats = h->stats;
+    A.av_out = h->rec_av;
     A.dt = dt; A.mode = mode;
     return A;
 }
@@ -270,6 +219,7 @@ void do_density_eos(asph_t* h, float dt, double simt, int bits, bool use_clock, 
         A.dep = h->dep; A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
         A.clock = use_clock ? h->clock : nullptr; A.simt = simt;
         A.stats = h->stats;
+        A.s12_out = h->rec_s12;
         A.dt = dt; A.bits = bits;
         ProfScope ps(h, ASPH_K_DENSITY_EOS);
         launch_density_eos(h->L, h->S, h->stream, A);
@@ -344,7 +294,9 @@ float sort_interval(const asph_config& c, const float u[3], float dt) {
     return std::fmin(std::fmax(std::fmin(dz_move, dh_move), dt), 15.0f);
 }
 
-}  // namespace
+}  // namespace asph_host
+
+using namespace asph_host;
 
 extern "C" {
 
@@ -888,7 +840,7 @@ DensArgs trial_dens_args(asph_t* h, float dt) {
     A.pos = h->pos; A.rho_old = h->rho; A.rho_new = h->rho2; A.thq = h->thq; A.prs = h->prs;
     A.vel = h->vel; A.cstart = h->cstart; A.keys_sorted = h->keys_sorted; A.rain = h->rain;
     A.dep = h->dep; A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
-    A.clock = nullptr; A.simt = h->simt; A.stats = nullptr;
+    A.clock = nullptr; A.simt = h->simt; A.stats = nullptr; A.s12_out = nullptr;
     A.dt = dt; A.bits = 0;
     return A;
 }

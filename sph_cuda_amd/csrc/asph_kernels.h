@@ -45,6 +45,7 @@ struct DensArgs {
     const double* clock;         // simulated time at the start of the step (device), or null:
     double simt;                 // then this value is used
     unsigned long long* stats;   // work counters (ASPH_C_DENS_*), or null
+    float2* s12_out;             // adjoint tape: the density sums (S1, S2) per particle, or null
     float dt;
     int bits;                    // bit 0: EOS + microphysics, bit 1: surface flux (if flagged)
 };
@@ -63,9 +64,43 @@ struct MomArgs {
     const float* disp;
     const uint32_t* disp_slots;
     unsigned long long* stats;   // work counters (ASPH_C_MOM_*), or null
+    float4* av_out;              // adjoint tape: the viscous acceleration sums (x mm), or null
     float dt;
     int mode;                    // 0: force evaluation, 1: step
 };
+
+// ---- adjoint (asph_adjoint.hip) ----------------------------------------------------------------
+// Forward values of one step, recorded while a segment is recomputed (device pointers).
+struct AdjTape {
+    const float4 *x0, *v0, *a0, *t0;   // state at the start of the step
+    const float* r0;                   // density at the start of the step
+    const int64_t* rain0;              // rain grid at the start of the step
+    const float4 *x1, *v1;             // after kick + drift; v1.w = p'
+    const float* r1;                   // density of the step
+    const float2* s12;                 // density sums (S1, S2)
+    const float4* t1;                  // theta, qv, qc after the EOS stage
+    const float4 *a1, *av;             // acceleration and viscous sums (x mm) of the step
+    const uint32_t* keys;              // sorted keys of the segment (listed cells)
+    const uint32_t* cstart;            // cell start table of the segment
+};
+// Adjoints of the state in the particle order of a segment.
+struct AdjState {
+    float4 *x, *v, *a, *t;
+    float* rho;
+    double* rain;                      // per cell
+};
+// Adjoints internal to one step's reverse.
+struct AdjWork {
+    float4 *Ab, *AVb, *x1b, *v1b, *t2b;
+    float *pb, *r1b;
+    float2* sb;
+    double* rb_cur;
+};
+// mode 1: a KDK step; mode 0: the force evaluation at t = 0 (no motion, no flux, rain merge).
+void launch_adj_step(const DevParams* P, int64_t N, hipStream_t s, const AdjTape& T, const AdjWork& W,
+                     const AdjState& E, const AdjState& Sst, double* qc0b, float dt, float Ts, float qs_sfc,
+                     float frac, int mode);
+void launch_adj_unperm(int64_t N, hipStream_t s, const uint32_t* perm, const AdjState& in, const AdjState& out);
 
 int particle_block_size();
 void launch_step_begin(hipStream_t s, float* disp, uint32_t* disp_slots, int reset, double* clock,

@@ -186,6 +186,7 @@ __device__ __forceinline__ void dens_epilogue(const DevParams& P, const DensArgs
     float rr = S2 > 1e-20f ? S1 / S2 : rref;
     float rnew = fminf(fmaxf(rr, 0.5f * rref), 2.0f * rref);
     A.rho_new[i] = rnew;
+    if (A.s12_out) A.s12_out[i] = make_float2(S1, S2);
     if (!(A.bits & 1)) return;
 
     float4 t = A.thq[i];
@@ -280,6 +281,7 @@ __device__ __forceinline__ void mom_epilogue(const DevParams& P, const MomArgs& 
     float thvr = thr * (1.0f + 0.608f * qv_ref(P, z));
     az = az + GRAV * ((thv - thvr) / thvr - ti.z);
     A.acc[i] = make_float4(ax, ay, az, 0.f);
+    if (A.av_out) A.av_out[i] = make_float4(avx, avy, avz, 0.f);
     float Pi = asphm::fpow(fmaxf(pres_of_exner(P, ex) + pri, 1.0f) / P0, P.rd_cp);
     float dth = -__builtin_fmaf(vi.z, avz, __builtin_fmaf(vi.y, avy, vi.x * avx)) / (CP * Pi);
     dth = fminf(fmaxf(dth, -P.dthdt_cap), P.dthdt_cap);

@@ -274,6 +274,19 @@ class Sim:
                 "asph_kernel_times")
         return {L.KERNEL_NAMES[k]: (float(ms[k]), int(cnt[k])) for k in range(L.K_COUNT)}
 
+    def adjoint_rain(self, nsteps: int, dt: float, checkpoint: int = 10) -> dict:
+        """Rain water at the end of an nsteps window and its gradient with respect to the initial
+        state (as set by set_state) and the autoconversion threshold (include/asph.h,
+        asph_adjoint_rain). Returns J [kg], dJ_dqc0, grad_theta (N,), grad_thq (N, 4) and times."""
+        a = L.AdjointArgs(nsteps, dt, checkpoint)
+        r = L.AdjointResult()
+        g = np.empty(self.n, np.float32)
+        g4 = np.empty((self.n, 4), np.float32)
+        L.check(self.lib.asph_adjoint_rain(self.h, C.byref(a), _fp(g), _fp(g4), C.byref(r)),
+                "asph_adjoint_rain")
+        return dict(J=r.J, dJ_dqc0=r.dJ_dqc0, grad_theta=g, grad_thq=g4, forward_s=r.forward_s,
+                    adjoint_s=r.adjoint_s, segments=r.segments)
+
     def tuning(self):
         t = L.Tuning()
         L.check(self.lib.asph_get_tuning(self.h, C.byref(t)), "asph_get_tuning")
