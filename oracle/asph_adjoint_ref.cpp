@@ -54,6 +54,7 @@ struct Tape {
     std::vector<uint32_t> keys;
     std::vector<int32_t> cell_range;
     double Ts = 0.0, qs = 0.0;   // surface skin temperature and saturation of the step
+    int mg[3] = {1, 1, 1};       // largest |current cell - listed cell| of the step per axis
 };
 
 struct Adj {
@@ -97,8 +98,28 @@ void sep(const asph::DevParams& P, const double* a, const double* b, double r[3]
     }
 }
 
-// Calls fn(j, own, rev) for the particles j listed within S + 1 cells of i's listed cell for which
-// (i, j) is a pair of i's sum (own) or (j, i) one of j's sum (rev), by the cell tests of the forward.
+// Cells added to the enumeration margin of for_pairs (a test setting: the gradient must not change).
+int g_extra_margin = 0;
+
+// Largest offset per axis between a particle's current cell (after the drift) and its listed cell.
+void drift_margin(const asph::DevParams& P, const Params<double>& Q, Tape& T) {
+    T.mg[0] = T.mg[1] = T.mg[2] = 0;
+    for (int64_t i = 0; i < P.N; ++i) {
+        const uint32_t key = T.keys[(size_t)i];
+        int c[3];
+        cur_cell(Q, &T.x1[4 * (size_t)i], c[0], c[1], c[2]);
+        const int l[3] = {(int)asph::compact1by2(key), (int)asph::compact1by2(key >> 1),
+                          (int)asph::compact1by2(key >> 2)};
+        T.mg[0] = std::max(T.mg[0], std::abs(wrapd(c[0] - l[0], P.nx)));
+        T.mg[1] = std::max(T.mg[1], std::abs(wrapd(c[1] - l[1], P.ny)));
+        T.mg[2] = std::max(T.mg[2], std::abs(c[2] - l[2]));
+    }
+}
+
+// Calls fn(j, own, rev) for the particles j for which (i, j) is a pair of i's sum (own: j listed in
+// the stencil of i's current cell) or (j, i) one of j's sum (rev: i listed in the stencil of j's
+// current cell). Both kinds are listed within S + mg cells of i's listed cell per axis (mg: the
+// step's largest current-to-listed offset); every cell in that range is visited once.
 template <class Fn>
 void for_pairs(const asph::DevParams& P, const Params<double>& Q, const Tape& T, int64_t i, Fn fn) {
     const uint32_t key = T.keys[i];
@@ -106,11 +127,13 @@ void for_pairs(const asph::DevParams& P, const Params<double>& Q, const Tape& T,
               lz = (int)asph::compact1by2(key >> 2);
     int ix, iy, iz;
     cur_cell(Q, &T.x1[4 * i], ix, iy, iz);
-    const int M = P.S + 1;
-    for (int cz = std::max(lz - M, 0); cz <= std::min(lz + M, P.nz - 1); ++cz)
-        for (int dy = -M; dy <= M; ++dy)
-            for (int dx = -M; dx <= M; ++dx) {
-                const int cx = Params<double>::wrap(lx + dx, P.nx), cy = Params<double>::wrap(ly + dy, P.ny);
+    const int Mx = P.S + T.mg[0] + g_extra_margin, My = P.S + T.mg[1] + g_extra_margin,
+              Mz = P.S + T.mg[2] + g_extra_margin;
+    const int nxs = std::min(2 * Mx + 1, P.nx), nys = std::min(2 * My + 1, P.ny);
+    for (int cz = std::max(lz - Mz, 0); cz <= std::min(lz + Mz, P.nz - 1); ++cz)
+        for (int oy = 0; oy < nys; ++oy)
+            for (int ox = 0; ox < nxs; ++ox) {
+                const int cx = Params<double>::wrap(lx - Mx + ox, P.nx), cy = Params<double>::wrap(ly - My + oy, P.ny);
                 const int64_t c = ((int64_t)cz * P.ny + cy) * P.nx + cx;
                 const bool own = in_stencil(P, ix, iy, iz, cx, cy, cz);
                 for (int32_t j = T.cell_range[2 * c]; j < T.cell_range[2 * c + 1]; ++j) {
@@ -300,6 +323,8 @@ void unperm(const std::vector<uint32_t>& perm, Adj& A) {
 
 extern "C" {
 
+void oracle_adjoint_set_extra_margin(int cells) { g_extra_margin = cells < 0 ? 0 : cells; }
+
 // FP64 host adjoint of asph_adjoint_rain (same schedule and objective). Inputs are the initial
 // state in the layout of asph_set_state (float64); outputs J, dJ/d(theta, qv, qc, -) per particle
 // in that order, and dJ/d(qc_auto).
@@ -331,6 +356,7 @@ int oracle_adjoint_rain(const asph_config* cfg, const double* pos4, const double
         o.rec_s12 = nullptr;
         o.eos_micro(dt);
         rec_mid(o, T);
+        drift_margin(P, Q, T);
         T.s12 = s12;
         o.rain_merge();
         o.rec_av = &av;
@@ -359,6 +385,7 @@ int oracle_adjoint_rain(const asph_config* cfg, const double* pos4, const double
         o.eos_micro(dt);
         // x1 and v1 are the positions and velocities after the drift; the flux changed thq only
         rec_mid(o, T);
+        drift_margin(P, Q, T);
         T.s12 = s12;
         o.rec_av = &av;
         o.momentum();

@@ -64,6 +64,7 @@ def load():
         "oracle_run": (C.c_int, [P, C.c_double, C.c_int64, C.c_float, C.c_int, DP]),
         "oracle_adjoint_rain": (C.c_int, [C.POINTER(Config), DP, DP, DP, DP, C.c_int64, C.c_float, C.c_int,
                                           DP, DP, DP]),
+        "oracle_adjoint_set_extra_margin": (None, [C.c_int]),
         "oracle_morton3": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
         "oracle_fexp": (C.c_float, [C.c_float]),
         "oracle_flog": (C.c_float, [C.c_float]),
@@ -251,10 +252,12 @@ class Oracle:
         return rc, dict(zip(keys, st.tolist()))
 
 
-def adjoint_rain_f64(cfg, st, nsteps, dt, checkpoint=10):
+def adjoint_rain_f64(cfg, st, nsteps, dt, checkpoint=10, extra_margin=0):
     """FP64 host adjoint (oracle/asph_adjoint_ref.cpp) of the rain objective of asph_adjoint_rain,
-    same schedule: returns (J [kg], dJ/d(theta, qv, qc, -) of shape (N, 4), dJ/d(qc_auto))."""
+    same schedule: returns (J [kg], dJ/d(theta, qv, qc, -) of shape (N, 4), dJ/d(qc_auto)).
+    extra_margin widens the pair enumeration by that many cells (the result must not change)."""
     lib = load()
+    lib.oracle_adjoint_set_extra_margin(int(extra_margin))
     cfg = cfg.copy()
     arrs = [np.ascontiguousarray(a, np.float64) for a in (st.pos, st.vel, st.thq, st.rho)]
     g = np.zeros((cfg.n, 4), np.float64)
@@ -262,6 +265,7 @@ def adjoint_rain_f64(cfg, st, nsteps, dt, checkpoint=10):
     dp = C.POINTER(C.c_double)
     rc = lib.oracle_adjoint_rain(C.byref(cfg), *[a.ctypes.data_as(dp) for a in arrs], nsteps, dt, checkpoint,
                                  C.byref(J), g.ctypes.data_as(dp), C.byref(gq))
+    lib.oracle_adjoint_set_extra_margin(0)
     if rc:
         raise RuntimeError("oracle_adjoint_rain failed")
     return J.value, g, gq.value

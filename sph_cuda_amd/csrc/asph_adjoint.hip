@@ -7,11 +7,12 @@
 // drift. Forward values come from the step's tape (recorded by the forward kernels while the
 // segment is recomputed from its checkpoint), so no stage recomputes a neighbour sum.
 //
-// Pair stages gather: the thread of particle i visits every particle j listed within S + 1 cells
-// of i's listed cell and adds the derivative terms of the pairs (i, j) of i's own sum (j listed in
-// the stencil of i's current cell) and (j, i) of j's sum (i listed in the stencil of j's current
-// cell), each with the forward's support test on the same arithmetic. The neighbour sets are
-// those of the forward, including its stale cell lists between sorts, and no atomics are needed.
+// Pair stages gather: the thread of particle i visits the particles j listed near i's listed cell
+// (cells culled against the support with the step's drift bound) and adds the derivative terms of
+// the pairs (i, j) of i's own sum (j listed in the stencil of i's current cell) and (j, i) of j's
+// sum (i listed in the stencil of j's current cell), each with the forward's support test on the
+// same arithmetic. The neighbour sets are those of the forward, including its stale cell lists
+// between sorts, and no atomics are needed.
 // Reductions into the rain grid and the autoconversion-threshold derivative are FP64.
 #include "asph_adjoint.h"
 #include "asph_kernels.h"
@@ -113,63 +114,128 @@ __global__ __launch_bounds__(TPB) void k_adj_mom_seed(const DevParams* __restric
     W.r1b[i] = E.rho[i];
 }
 
+// ---- the candidates of a particle's pair stages ------------------------------------------------------
+// The pairs of i's reverse are (i, j) of i's own sum (j listed in the stencil of i's current cell)
+// and (j, i) of j's sum (i listed in the stencil of j's current cell). A particle listed in cell c
+// lies within [c - d, c + 1 + d] per axis (d: the step's drift bound since the sort, with its
+// rounding margin), so its current cell is within c +- (1 + floor d), and both kinds of partner
+// are listed within M = S + 1 + floor(d) cells of i's listed cell. The walk visits those cells,
+// skips every cell whose box [c - d, c + 1 + d] (nearest periodic image) lies outside the support
+// around i (lim widened by 1e-4 relative), and calls f(j, own) for each particle listed in the rest,
+// own telling whether the cell is in the stencil of i's current cell.
+struct Around {
+    float xc, yc, zc;   // position [cells]
+    int lx, ly, lz;     // listed cell
+    int ix, iy, iz;     // current cell
+};
+
+__device__ __forceinline__ Around around(const DevParams& P, float4 x, uint32_t key) {
+    Around a;
+    a.xc = x.x * P.inv_dx;
+    a.yc = x.y * P.inv_dy;
+    a.zc = x.z * P.inv_dz;
+    listed_cell(key, a.lx, a.ly, a.lz);
+    cell_trunc(P, x.x, x.y, x.z, a.ix, a.iy, a.iz);
+    return a;
+}
+
+// Distance [cells] from t (the offset of a point from a cell's centre) to the cell's box widened
+// by d, over the nearest periodic image when n > 0.
+__device__ __forceinline__ float box_gap(float t, float d, int n) {
+    if (n > 0) {
+        const float hn = 0.5f * (float)n;
+        if (t > hn) t -= (float)n;
+        else if (t < -hn) t += (float)n;
+    }
+    return fmaxf(0.f, fabsf(t) - 0.5f - d);
+}
+
+template <class F>
+__device__ __forceinline__ void walk_partners(const DevParams& P, const uint32_t* __restrict__ cstart,
+                                              const float* __restrict__ dl, const Around& a, F f) {
+    const float limc = P.lim * 1.0001f;
+    const int S = P.S;
+    const float d0 = dl[0], d1 = dl[1], d2 = dl[2];
+    const int Mx = S + 1 + (int)d0, My = S + 1 + (int)d1, Mz = S + 1 + (int)d2;
+    const int nxs = min(2 * Mx + 1, P.nx), nys = min(2 * My + 1, P.ny);
+    const int za = max(a.lz - Mz, 0), zb = min(a.lz + Mz, P.nz - 1);
+    for (int cz = za; cz <= zb; ++cz) {
+        const float ez = box_gap(a.zc - ((float)cz + 0.5f), d2, 0);
+        const float ez2 = ez * ez;
+        if (ez2 >= limc) continue;
+        for (int oy = 0; oy < nys; ++oy) {
+            const int cyu = a.ly - My + oy;
+            const float ey = box_gap(a.yc - ((float)cyu + 0.5f), d1, P.ny);
+            const float eyz = __builtin_fmaf(ey, ey, ez2);
+            if (eyz >= limc) continue;
+            const int cy = wrapi(cyu, P.ny);
+            for (int ox = 0; ox < nxs; ++ox) {
+                const int cxu = a.lx - Mx + ox;
+                const float ex = box_gap(a.xc - ((float)cxu + 0.5f), d0, P.nx);
+                if (__builtin_fmaf(ex, ex, eyz) >= limc) continue;
+                const int cx = wrapi(cxu, P.nx);
+                const bool own = in_stencil(P, a.ix, a.iy, a.iz, cx, cy, cz);
+                const uint32_t key = morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+                const uint32_t j0 = cstart[key], j1 = cstart[key + 1];
+                for (uint32_t j = j0; j < j1; ++j) f(j, own);
+            }
+        }
+    }
+}
+
+// The support test shared by both orientations of a pair (the forward's arithmetic; negating r is
+// exact, so both orientations square to the same value) and the reverse-role stencil test.
+__device__ __forceinline__ bool in_support(const DevParams& P, float rx, float ry, float rz) {
+    const float dxm = rx * P.inv_dx, dym = ry * P.inv_dy, dzm = rz * P.inv_dz;
+    const float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+    return r2 < P.lim;
+}
+
+__device__ __forceinline__ bool rev_role(const DevParams& P, float4 xj, const Around& a) {
+    int jx, jy, jz;
+    cell_trunc(P, xj.x, xj.y, xj.z, jx, jy, jz);
+    return in_stencil(P, jx, jy, jz, a.lx, a.ly, a.lz);
+}
+
 // ---- momentum pairs (gather) -------------------------------------------------------------------
 __global__ __launch_bounds__(TPB) void k_adj_mom_pairs(const DevParams* __restrict__ Pp, AdjTape T, AdjWork W) {
     const DevParams& P = *Pp;
     const int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= P.N) return;
-    const int S = P.S;
     const float4 xi = T.x1[i], vi4 = T.v1[i];
     const float vi[3] = {vi4.x, vi4.y, vi4.z};
     const float rho_i_raw = T.r1[i];
     const float rho_i = fmaxf(rho_i_raw, 1e-3f), p_i = vi4.w;
     const float4 Ai4 = W.Ab[i], AVi4 = W.AVb[i];
     const float Ai[3] = {Ai4.x, Ai4.y, Ai4.z}, AVi[3] = {AVi4.x, AVi4.y, AVi4.z};
-    int lx, ly, lz, ix, iy, iz;
-    listed_cell(T.keys[i], lx, ly, lz);
-    cell_trunc(P, xi.x, xi.y, xi.z, ix, iy, iz);
+    const Around a = around(P, xi, T.keys[i]);
     float xb[3] = {0.f, 0.f, 0.f}, vb[3] = {0.f, 0.f, 0.f}, rb = 0.f, pb = 0.f;
-    const int M = S + 1;
-    for (int cz = max(lz - M, 0); cz <= min(lz + M, P.nz - 1); ++cz)
-        for (int dy = -M; dy <= M; ++dy) {
-            const int cy = wrapi(ly + dy, P.ny);
-            for (int dx = -M; dx <= M; ++dx) {
-                const int cx = wrapi(lx + dx, P.nx);
-                const uint32_t key = morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
-                const bool own = in_stencil(P, ix, iy, iz, cx, cy, cz);
-                for (uint32_t j = T.cstart[key]; j < T.cstart[key + 1]; ++j) {
-                    const float4 xj = T.x1[j];
-                    int jx, jy, jz;
-                    cell_trunc(P, xj.x, xj.y, xj.z, jx, jy, jz);
-                    const bool rev = in_stencil(P, jx, jy, jz, lx, ly, lz);
-                    if (!own && !rev) continue;
-                    const float4 vj4 = T.v1[j];
-                    const float vj[3] = {vj4.x, vj4.y, vj4.z};
-                    const float rho_j = fmaxf(T.r1[j], 1e-3f), p_j = vj4.w;
-                    adj::PairBar<float> b;
-                    if (own) {
-                        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                        min_image_adj(P, rx, ry);
-                        if (adj::mom_pair_vjp<float>(P, rx, ry, rz, vi, vj, rho_i, rho_j, p_i, p_j, Ai, AVi, b)) {
-                            for (int a = 0; a < 3; ++a) { xb[a] += b.r[a]; vb[a] += b.vd[a]; }
-                            rb += b.rho_o;
-                            pb += b.p_o;
-                        }
-                    }
-                    if (rev) {
-                        const float4 Aj4 = W.Ab[j], AVj4 = W.AVb[j];
-                        const float Aj[3] = {Aj4.x, Aj4.y, Aj4.z}, AVj[3] = {AVj4.x, AVj4.y, AVj4.z};
-                        float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
-                        min_image_adj(P, rx, ry);
-                        if (adj::mom_pair_vjp<float>(P, rx, ry, rz, vj, vi, rho_j, rho_i, p_j, p_i, Aj, AVj, b)) {
-                            for (int a = 0; a < 3; ++a) { xb[a] -= b.r[a]; vb[a] -= b.vd[a]; }
-                            rb += b.rho_n;
-                            pb += b.p_n;
-                        }
-                    }
-                }
+    walk_partners(P, T.cstart, T.delta, a, [&](uint32_t j, bool own) {
+        const float4 xj = T.x1[j];
+        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        min_image_adj(P, rx, ry);
+        if (!in_support(P, rx, ry, rz)) return;
+        const bool rev = rev_role(P, xj, a);
+        if (!own && !rev) return;
+        const float4 vj4 = T.v1[j];
+        const float vj[3] = {vj4.x, vj4.y, vj4.z};
+        const float rho_j = fmaxf(T.r1[j], 1e-3f), p_j = vj4.w;
+        adj::PairBar<float> b;
+        if (own && adj::mom_pair_vjp<float>(P, rx, ry, rz, vi, vj, rho_i, rho_j, p_i, p_j, Ai, AVi, b)) {
+            for (int c = 0; c < 3; ++c) { xb[c] += b.r[c]; vb[c] += b.vd[c]; }
+            rb += b.rho_o;
+            pb += b.p_o;
+        }
+        if (rev) {
+            const float4 Aj4 = W.Ab[j], AVj4 = W.AVb[j];
+            const float Aj[3] = {Aj4.x, Aj4.y, Aj4.z}, AVj[3] = {AVj4.x, AVj4.y, AVj4.z};
+            if (adj::mom_pair_vjp<float>(P, -rx, -ry, -rz, vj, vi, rho_j, rho_i, p_j, p_i, Aj, AVj, b)) {
+                for (int c = 0; c < 3; ++c) { xb[c] -= b.r[c]; vb[c] -= b.vd[c]; }
+                rb += b.rho_n;
+                pb += b.p_n;
             }
         }
+    });
     float4 x1b = W.x1b[i], v1b = W.v1b[i];
     W.x1b[i] = make_float4(x1b.x + xb[0], x1b.y + xb[1], x1b.z + xb[2], 0.f);
     W.v1b[i] = make_float4(v1b.x + vb[0], v1b.y + vb[1], v1b.z + vb[2], 0.f);
@@ -221,56 +287,37 @@ __global__ __launch_bounds__(TPB) void k_adj_dens_seed(const DevParams* __restri
 }
 
 // ---- density pairs (gather) ---------------------------------------------------------------------
+// W and its gradient are even and odd in r, so one evaluation serves both orientations.
 __global__ __launch_bounds__(TPB) void k_adj_dens_pairs(const DevParams* __restrict__ Pp, AdjTape T, AdjWork W,
                                                         AdjState Sst) {
     const DevParams& P = *Pp;
     const int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= P.N) return;
-    const int S = P.S;
     const float4 xi = T.x1[i];
     const float r0i_raw = T.r0[i];
     const float inv_r0i = 1.0f / fmaxf(r0i_raw, 1e-3f);
     const float2 sbi = W.sb[i];
-    int lx, ly, lz, ix, iy, iz;
-    listed_cell(T.keys[i], lx, ly, lz);
-    cell_trunc(P, xi.x, xi.y, xi.z, ix, iy, iz);
+    const Around a = around(P, xi, T.keys[i]);
     float xb[3] = {0.f, 0.f, 0.f}, rb = 0.f;
-    const int M = S + 1;
-    for (int cz = max(lz - M, 0); cz <= min(lz + M, P.nz - 1); ++cz)
-        for (int dy = -M; dy <= M; ++dy) {
-            const int cy = wrapi(ly + dy, P.ny);
-            for (int dx = -M; dx <= M; ++dx) {
-                const int cx = wrapi(lx + dx, P.nx);
-                const uint32_t key = morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
-                const bool own = in_stencil(P, ix, iy, iz, cx, cy, cz);
-                for (uint32_t j = T.cstart[key]; j < T.cstart[key + 1]; ++j) {
-                    const float4 xj = T.x1[j];
-                    int jx, jy, jz;
-                    cell_trunc(P, xj.x, xj.y, xj.z, jx, jy, jz);
-                    const bool rev = in_stencil(P, jx, jy, jz, lx, ly, lz);
-                    if (!own && !rev) continue;
-                    float Wv, gw[3];
-                    if (own) {
-                        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
-                        min_image_adj(P, rx, ry);
-                        if (adj::dens_pair<float>(P, rx, ry, rz, Wv, gw)) {
-                            const float wb = sbi.x + sbi.y * (1.0f / fmaxf(T.r0[j], 1e-3f));
-                            for (int a = 0; a < 3; ++a) xb[a] += wb * gw[a];
-                        }
-                    }
-                    if (rev) {
-                        float rx = xj.x - xi.x, ry = xj.y - xi.y, rz = xj.z - xi.z;
-                        min_image_adj(P, rx, ry);
-                        if (adj::dens_pair<float>(P, rx, ry, rz, Wv, gw)) {
-                            const float2 sbj = W.sb[j];
-                            const float wb = sbj.x + sbj.y * inv_r0i;
-                            for (int a = 0; a < 3; ++a) xb[a] -= wb * gw[a];
-                            rb += sbj.y * Wv * (-inv_r0i * inv_r0i);
-                        }
-                    }
-                }
-            }
+    walk_partners(P, T.cstart, T.delta, a, [&](uint32_t j, bool own) {
+        const float4 xj = T.x1[j];
+        float rx = xi.x - xj.x, ry = xi.y - xj.y, rz = xi.z - xj.z;
+        min_image_adj(P, rx, ry);
+        if (!in_support(P, rx, ry, rz)) return;
+        const bool rev = rev_role(P, xj, a);
+        if (!own && !rev) return;
+        float Wv, gw[3];
+        if (!adj::dens_pair<float>(P, rx, ry, rz, Wv, gw)) return;
+        float wb = 0.f;
+        if (own) wb = sbi.x + sbi.y * (1.0f / fmaxf(T.r0[j], 1e-3f));
+        if (rev) {
+            // (j, i): r_ji = -r_ij, so the gradient flips sign and x_i enters with a minus
+            const float2 sbj = W.sb[j];
+            wb += sbj.x + sbj.y * inv_r0i;
+            rb += sbj.y * Wv * (-inv_r0i * inv_r0i);
         }
+        for (int c = 0; c < 3; ++c) xb[c] += wb * gw[c];
+    });
     float4 x1b = W.x1b[i];
     W.x1b[i] = make_float4(x1b.x + xb[0], x1b.y + xb[1], x1b.z + xb[2], 0.f);
     Sst.rho[i] = r0i_raw > 1e-3f ? rb : 0.f;
@@ -299,6 +346,21 @@ __global__ __launch_bounds__(TPB) void k_adj_kd(const DevParams* __restrict__ Pp
     Sst.x[i] = make_float4(xb.x, xb.y, zb, 0.f);
     Sst.v[i] = make_float4(vpx, vpy, vpz, 0.f);
     Sst.a[i] = make_float4(hdt * vpx, hdt * vpy, hdt * vpz, 0.f);
+}
+
+// ---- the drift bound of a step's neighbour stages ------------------------------------------------
+// The same fold as the forward kernels' prologue: the drift since the sort plus the step's
+// per-slot maxima, plus the 1e-3 cell rounding margin of the culling bounds.
+__global__ __launch_bounds__(64) void k_adj_delta(const float* __restrict__ disp, const uint32_t* __restrict__ slots,
+                                                 float* __restrict__ out) {
+    const int t = threadIdx.x;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        float v = __uint_as_float(slots[c * 64 + t]);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = fmaxf(v, __shfl_xor(v, o));
+        if (t == 0) out[c] = disp[c] + v + 1e-3f;
+    }
 }
 
 // ---- undo a sort: adjoints of the gathered arrays scatter back through the permutation ----------
@@ -330,6 +392,10 @@ void launch_adj_step(const DevParams* P, int64_t N, hipStream_t s, const AdjTape
     k_adj_dens_seed<<<g, TPB, 0, s>>>(P, T, W);
     k_adj_dens_pairs<<<g, TPB, 0, s>>>(P, T, W, Sst);
     k_adj_kd<<<g, TPB, 0, s>>>(P, T, W, Sst, dt, mode);
+}
+
+void launch_adj_delta(hipStream_t s, const float* disp, const uint32_t* slots, float* out) {
+    k_adj_delta<<<1, 64, 0, s>>>(disp, slots, out);
 }
 
 void launch_adj_unperm(int64_t N, hipStream_t s, const uint32_t* perm, const AdjState& in, const AdjState& out) {

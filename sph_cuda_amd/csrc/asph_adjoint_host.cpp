@@ -38,7 +38,7 @@ struct Checkpoint {
 
 struct TapeStep {
     float4 *x0, *v0, *a0, *t0, *x1, *v1, *t1, *a1, *av;
-    float *r0, *r1;
+    float *r0, *r1, *delta;
     float2* s12;
     int64_t* rain0;
 };
@@ -107,7 +107,7 @@ AdjTape tape_view(asph_t* h, const TapeStep& t) {
     AdjTape T;
     T.x0 = t.x0; T.v0 = t.v0; T.a0 = t.a0; T.t0 = t.t0; T.r0 = t.r0; T.rain0 = t.rain0;
     T.x1 = t.x1; T.v1 = t.v1; T.r1 = t.r1; T.s12 = t.s12; T.t1 = t.t1; T.a1 = t.a1; T.av = t.av;
-    T.keys = h->keys_sorted; T.cstart = h->cstart;
+    T.keys = h->keys_sorted; T.cstart = h->cstart; T.delta = t.delta;
     return T;
 }
 
@@ -147,8 +147,8 @@ int asph_adjoint_rain(asph_t* h, const asph_adjoint_args* a, float* dJ_dtheta0, 
         t.x0 = A.get<float4>(N); t.v0 = A.get<float4>(N); t.a0 = A.get<float4>(N); t.t0 = A.get<float4>(N);
         t.x1 = A.get<float4>(N); t.v1 = A.get<float4>(N); t.t1 = A.get<float4>(N); t.a1 = A.get<float4>(N);
         t.av = A.get<float4>(N); t.r0 = A.get<float>(N); t.r1 = A.get<float>(N); t.s12 = A.get<float2>(N);
-        t.rain0 = A.get<int64_t>(N);
-        if (!t.x0 || !t.v0 || !t.a0 || !t.t0 || !t.x1 || !t.v1 || !t.t1 || !t.a1 || !t.av || !t.r0 || !t.r1 ||
+        t.rain0 = A.get<int64_t>(N); t.delta = A.get<float>(4);
+        if (!t.delta || !t.x0 || !t.v0 || !t.a0 || !t.t0 || !t.x1 || !t.v1 || !t.t1 || !t.a1 || !t.av || !t.r0 || !t.r1 ||
             !t.s12 || !t.rain0)
             return fail("adjoint: out of memory (tape)");
     }
@@ -228,6 +228,7 @@ int asph_adjoint_rain(asph_t* h, const asph_adjoint_args* a, float* dJ_dtheta0, 
             h->rec_av = tinit.av;
             launch_momentum_kdk(h->L, h->S, s, mom_args(h, dt, 0));
             h->rec_av = nullptr;
+            launch_adj_delta(s, h->disp, h->disp_slots, tinit.delta);
             ACHK(hipMemcpyAsync(tinit.a1, h->acc, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
             h->initialized = true;
         }
@@ -245,6 +246,7 @@ int asph_adjoint_rain(asph_t* h, const asph_adjoint_args* a, float* dJ_dtheta0, 
             h->rec_av = t.av;
             do_momentum_kdk(h, dt, true);
             h->rec_av = nullptr;
+            launch_adj_delta(s, h->disp, h->disp_slots, t.delta);
             ACHK(hipMemcpyAsync(t.a1, h->acc, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
             do_rain(h, dt, true);
             h->simt += dt;

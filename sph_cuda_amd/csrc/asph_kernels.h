@@ -82,6 +82,7 @@ struct AdjTape {
     const float4 *a1, *av;             // acceleration and viscous sums (x mm) of the step
     const uint32_t* keys;              // sorted keys of the segment (listed cells)
     const uint32_t* cstart;            // cell start table of the segment
+    const float* delta;                // drift bound of the step's neighbour stages [cells per axis]
 };
 // Adjoints of the state in the particle order of a segment.
 struct AdjState {
@@ -100,6 +101,7 @@ struct AdjWork {
 void launch_adj_step(const DevParams* P, int64_t N, hipStream_t s, const AdjTape& T, const AdjWork& W,
                      const AdjState& E, const AdjState& Sst, double* qc0b, float dt, float Ts, float qs_sfc,
                      float frac, int mode);
+void launch_adj_delta(hipStream_t s, const float* disp, const uint32_t* slots, float* out);
 void launch_adj_unperm(int64_t N, hipStream_t s, const uint32_t* perm, const AdjState& in, const AdjState& out);
 
 int particle_block_size();

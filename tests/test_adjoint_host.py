@@ -59,3 +59,50 @@ def test_threshold_gradient_matches_central_differences(case):
           - rain_objective_f64(cm, st.pos, st.vel, th, st.rho, NSTEPS, dt, CK)) / (2 * de)
     print(f"[case] qc0: FD {fd:.9e} adjoint {gq:.9e}", flush=True)
     assert gq < 0 and abs(gq - fd) <= 2e-6 * abs(fd)
+
+
+def _windy(cfg, st, cells, nsteps, dt):
+    """The state with a uniform horizontal wind that carries every particle `cells` cells along x
+    (and half of that along y) over the window: listed and current cells drift apart by more than
+    one cell between sorts, the regime of the forward's global-table walk."""
+    vel = st.vel.copy()
+    vel[:, 0] += np.float32(cells * cfg.dx / (nsteps * dt))
+    vel[:, 1] += np.float32(0.5 * cells * cfg.dy / (nsteps * dt))
+    return s.State(st.pos, vel, st.thq, st.rho)
+
+
+def test_pair_enumeration_is_complete_with_stale_lists():
+    """With drift beyond a cell between sorts, widening the pair walk by 3 cells per axis (the whole
+    x-y lattice here) finds no further pair: the gradient changes only by summation order."""
+    cfg, ic, dt = small_config("adj", (12, 12, 10))
+    n, ck = 6, 6
+    st = _windy(cfg, s.ic_generate(cfg, ic), 2.5, n, dt)
+    J, g, gq = adjoint_rain_f64(cfg, st, n, dt, ck)
+    J2, g2, gq2 = adjoint_rain_f64(cfg, st, n, dt, ck, extra_margin=3)
+    assert J == J2
+    assert np.max(np.abs(g - g2)) <= 1e-12 * np.max(np.abs(g))
+    assert abs(gq - gq2) <= 1e-12 * abs(gq)
+
+
+def test_gradient_with_stale_lists_beyond_one_cell():
+    """Finite differences in that regime. A pair enters or leaves the stencil when a particle crosses
+    a cell face, so J has small jumps there and a central difference carries O(jump / eps) noise
+    (3e-5 relative at eps 1e-5 to 1e-4 on this case, against 2e-7 without wind): the bound is wider."""
+    cfg, ic, dt = small_config("adj", (12, 12, 10))
+    n, ck = 6, 6   # one segment: no re-sort inside the window
+    st = _windy(cfg, s.ic_generate(cfg, ic), 2.5, n, dt)
+    J, g, _ = adjoint_rain_f64(cfg, st, n, dt, ck)
+    rng = np.random.default_rng(4100)
+    base = st.thq.astype(np.float64)
+    eps = 1e-4
+    for k in range(2):
+        d = rng.standard_normal(cfg.n)
+        tp, tm = base.copy(), base.copy()
+        tp[:, 0] += eps * d
+        tm[:, 0] -= eps * d
+        fd = (rain_objective_f64(cfg, st.pos, st.vel, tp, st.rho, n, dt, ck)
+              - rain_objective_f64(cfg, st.pos, st.vel, tm, st.rho, n, dt, ck)) / (2 * eps)
+        ad = float(np.dot(g[:, 0], d))
+        scale = np.linalg.norm(g[:, 0]) * np.linalg.norm(d)
+        print(f"[case] windy dir {k}: FD {fd:.9e} adjoint {ad:.9e}", flush=True)
+        assert abs(ad - fd) <= 2e-4 * max(abs(fd), 1e-2 * scale)

@@ -1,0 +1,145 @@
+"""The device adjoint (asph_adjoint_rain): against the FP64 host adjoint (same schedule, same pair
+sets, FP64 throughout), against central differences of the FP64 oracle, through the torch.autograd
+binding, and for its side effects on the handle.
+
+Tolerances. The device runs the model in FP32 and the references in FP64, so the two trajectories
+part by about 1e-7 relative per step; the gradients then differ by what the FP32 model's own
+derivative differs from the FP64 one. Measured on the 20x20x16 case: 6e-5 to 1e-4 normwise per
+component against the host adjoint, 1.1e-4 normwise against finite differences, 5e-5 on
+dJ/dqc0. The bounds below leave a factor of about five.
+"""
+import numpy as np
+import pytest
+import torch
+
+import sph_cuda_amd as s
+from sph_cuda_amd import _lib as L
+from helpers import small_config
+from oracle.oracle import adjoint_rain_f64, rain_objective_f64
+from sph_cuda_amd.autograd import rain_objective
+
+pytestmark = pytest.mark.gpu
+
+SHAPE, NSTEPS, CK = (20, 20, 16), 8, 4
+
+
+def _normwise(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def _windy(cfg, st, cells, nsteps, dt):
+    """Uniform wind that carries every particle `cells` cells along x (half along y) over the window."""
+    vel = st.vel.copy()
+    vel[:, 0] += np.float32(cells * cfg.dx / (nsteps * dt))
+    vel[:, 1] += np.float32(0.5 * cells * cfg.dy / (nsteps * dt))
+    return s.State(st.pos, vel, st.thq, st.rho)
+
+
+@pytest.fixture(scope="module")
+def case():
+    cfg, ic, dt = small_config("adj", SHAPE)
+    st = s.ic_generate(cfg, ic)
+    print(f"[case] adjoint {SHAPE} steps {NSTEPS} checkpoint {CK}", flush=True)
+    with s.Sim(cfg) as sim:
+        sim.set_state(st)
+        r = sim.adjoint_rain(NSTEPS, dt, CK)
+    return cfg, dt, st, r, adjoint_rain_f64(cfg, st, NSTEPS, dt, CK)
+
+
+def test_matches_host_adjoint(case):
+    cfg, dt, st, r, (J64, g64, gq64) = case
+    g = r["grad_thq"].astype(np.float64)
+    rel = [_normwise(g[:, c], g64[:, c]) for c in range(3)]
+    print(f"[case] normwise theta/qv/qc {rel}, J {r['J']:.9e} vs {J64:.9e}, "
+          f"qc0 {r['dJ_dqc0']:.6e} vs {gq64:.6e}", flush=True)
+    assert r["segments"] == (NSTEPS + CK - 1) // CK
+    assert np.all(np.isfinite(g))
+    assert max(rel) <= 5e-4
+    assert abs(r["J"] - J64) <= 5e-3 * J64
+    assert abs(r["dJ_dqc0"] - gq64) <= 5e-4 * abs(gq64)
+    assert np.array_equal(r["grad_theta"], r["grad_thq"][:, 0])
+
+
+def test_matches_central_differences(case):
+    cfg, dt, st, r, _ = case
+    g = r["grad_theta"].astype(np.float64)
+    rng = np.random.default_rng(4000)
+    base = st.thq.astype(np.float64)
+    eps = 1e-3
+    fds, ads = [], []
+    for k in range(3):
+        d = rng.standard_normal(cfg.n)
+        tp, tm = base.copy(), base.copy()
+        tp[:, 0] += eps * d
+        tm[:, 0] -= eps * d
+        fd = (rain_objective_f64(cfg, st.pos, st.vel, tp, st.rho, NSTEPS, dt, CK)
+              - rain_objective_f64(cfg, st.pos, st.vel, tm, st.rho, NSTEPS, dt, CK)) / (2 * eps)
+        fds.append(fd)
+        ads.append(float(g @ d))
+        print(f"[case] dir {k}: FD {fd:.6e} adjoint {ads[-1]:.6e}", flush=True)
+    fds, ads = np.array(fds), np.array(ads)
+    assert _normwise(ads, fds) <= 1e-3
+
+
+def test_stale_lists_beyond_one_cell():
+    """One segment with a wind that moves every particle 2.5 cells from its listed cell: the forward
+    leaves the staged bricks for the global cell table and the reverse walk widens its margin with
+    the recorded drift bound. The pair sets must stay those of the host adjoint."""
+    cfg, ic, dt = small_config("adj", SHAPE)
+    n = 8
+    st = _windy(cfg, s.ic_generate(cfg, ic), 2.5, n, dt)
+    print(f"[case] windy adjoint {SHAPE} steps {n}", flush=True)
+    with s.Sim(cfg) as sim:
+        sim.set_state(st)
+        r = sim.adjoint_rain(n, dt, n)
+    J64, g64, gq64 = adjoint_rain_f64(cfg, st, n, dt, n)
+    g = r["grad_thq"].astype(np.float64)
+    rel = [_normwise(g[:, c], g64[:, c]) for c in range(3)]
+    print(f"[case] windy normwise theta/qv/qc {rel}, qc0 {r['dJ_dqc0']:.6e} vs {gq64:.6e}", flush=True)
+    assert max(rel) <= 1e-3
+    assert abs(r["dJ_dqc0"] - gq64) <= 1e-3 * abs(gq64)
+
+
+def test_handle_state_is_restored(case):
+    cfg, dt, st, _, _ = case
+    with s.Sim(cfg) as sim, s.Sim(cfg) as ref:
+        sim.set_state(st)
+        ref.set_state(st)
+        sim.adjoint_rain(3, dt, 2)
+        a, b = sim.get_state(), ref.get_state()
+        for f in ("pos", "vel", "thq", "rho"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f
+        for h in (sim, ref):
+            h.sort()
+            h.force(dt)
+            h.step(dt, 0.0, do_sort=False)
+        a, b = sim.get_state(), ref.get_state()
+        for f in ("pos", "vel", "thq", "rho"):
+            assert np.array_equal(getattr(a, f), getattr(b, f)), f
+
+
+def test_autograd_binding(case):
+    cfg, dt, st, r, _ = case
+    theta0 = torch.tensor(st.thq[:, 0], dtype=torch.float32, requires_grad=True)
+    qc0 = torch.tensor(float(cfg.qc_auto), dtype=torch.float64, requires_grad=True)
+    J = rain_objective(theta0, qc0, st, cfg, NSTEPS, dt, CK)
+    assert J.dtype == torch.float64 and abs(float(J) - r["J"]) <= 1e-12 * r["J"]
+    (3.0 * J).backward()
+    assert theta0.grad is not None and qc0.grad is not None
+    # a second run of the adjoint: its FP64 atomics (rain grid, qc0) may sum in another order
+    g3 = 3.0 * r["grad_theta"].astype(np.float64)
+    np.testing.assert_allclose(theta0.grad.numpy(), g3, rtol=1e-4, atol=1e-6 * np.max(np.abs(g3)))
+    assert float(qc0.grad) == pytest.approx(3.0 * r["dJ_dqc0"], rel=1e-6)
+
+
+def test_rejects_what_it_cannot_differentiate(case):
+    cfg8, ic8, dt = small_config("adj", (8, 8, 8))   # narrower than 2 * stencil + 3
+    with s.Sim(cfg8) as sim:
+        sim.set_state(s.ic_generate(cfg8, ic8))
+        with pytest.raises(L.AsphError, match="nx, ny"):
+            sim.adjoint_rain(2, dt, 2)
+    cfg, dt, st, _, _ = case
+    with s.Sim(cfg) as sim:
+        sim.set_state(st)
+        with pytest.raises(L.AsphError, match="nsteps"):
+            sim.adjoint_rain(0, dt, 2)
