@@ -64,7 +64,12 @@ def load():
         "oracle_run": (C.c_int, [P, C.c_double, C.c_int64, C.c_float, C.c_int, DP]),
         "oracle_adjoint_rain": (C.c_int, [C.POINTER(Config), DP, DP, DP, DP, C.c_int64, C.c_float, C.c_int,
                                           DP, DP, DP]),
+        "oracle_adjoint_rain_f32_trajectory": (C.c_int, [C.POINTER(Config), DP, DP, DP, DP, C.c_int64, C.c_float,
+                                                          C.c_int, DP, DP, DP]),
         "oracle_adjoint_set_extra_margin": (None, [C.c_int]),
+        "oracle_adjoint_set_float_stages": (None, [C.c_int]),
+        "oracle_check_mom_pair_fused": (C.c_double, [C.POINTER(Config), C.c_int, C.c_uint64, C.c_int]),
+        "oracle_check_mom_vjp_rev": (C.c_double, [C.POINTER(Config), C.c_int, C.c_uint64, C.c_int, C.c_int]),
         "oracle_morton3": (C.c_uint32, [C.c_uint32, C.c_uint32, C.c_uint32]),
         "oracle_fexp": (C.c_float, [C.c_float]),
         "oracle_flog": (C.c_float, [C.c_float]),
@@ -252,20 +257,27 @@ class Oracle:
         return rc, dict(zip(keys, st.tolist()))
 
 
-def adjoint_rain_f64(cfg, st, nsteps, dt, checkpoint=10, extra_margin=0):
+def adjoint_rain_f64(cfg, st, nsteps, dt, checkpoint=10, extra_margin=0, trajectory="f64", float_stages=0):
     """FP64 host adjoint (oracle/asph_adjoint_ref.cpp) of the rain objective of asph_adjoint_rain,
     same schedule: returns (J [kg], dJ/d(theta, qv, qc, -) of shape (N, 4), dJ/d(qc_auto)).
-    extra_margin widens the pair enumeration by that many cells (the result must not change)."""
+    extra_margin widens the pair enumeration by that many cells (the result must not change).
+    trajectory "f32" runs the reverse (in double) along the FP32 oracle's forward instead.
+    float_stages (bits: 1 momentum pairs, 2 density pairs, 4 EOS and microphysics, 8 state adjoints,
+    16 momentum epilogue, 32 density ratio)
+    evaluates those reverse stages in float, as the device does."""
     lib = load()
     lib.oracle_adjoint_set_extra_margin(int(extra_margin))
+    lib.oracle_adjoint_set_float_stages(int(float_stages))
     cfg = cfg.copy()
     arrs = [np.ascontiguousarray(a, np.float64) for a in (st.pos, st.vel, st.thq, st.rho)]
     g = np.zeros((cfg.n, 4), np.float64)
     J, gq = C.c_double(), C.c_double()
     dp = C.POINTER(C.c_double)
-    rc = lib.oracle_adjoint_rain(C.byref(cfg), *[a.ctypes.data_as(dp) for a in arrs], nsteps, dt, checkpoint,
-                                 C.byref(J), g.ctypes.data_as(dp), C.byref(gq))
+    fn = {"f64": lib.oracle_adjoint_rain, "f32": lib.oracle_adjoint_rain_f32_trajectory}[trajectory]
+    rc = fn(C.byref(cfg), *[a.ctypes.data_as(dp) for a in arrs], nsteps, dt, checkpoint,
+            C.byref(J), g.ctypes.data_as(dp), C.byref(gq))
     lib.oracle_adjoint_set_extra_margin(0)
+    lib.oracle_adjoint_set_float_stages(0)
     if rc:
         raise RuntimeError("oracle_adjoint_rain failed")
     return J.value, g, gq.value

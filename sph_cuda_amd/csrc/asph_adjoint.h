@@ -25,12 +25,22 @@ template <> struct Fn<float> {
     static ASPH_DH float pow(float x, float y) { return asphm::fpow(x, y); }
     static ASPH_DH float sqrt(float x) { return sqrtf(x); }
     static ASPH_DH float fma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+    // reciprocal for the reverse's pair terms, which need no correctly rounded division: the
+    // hardware reciprocal (1 ulp) on the device, a division on the host
+    static ASPH_DH float rcp(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        return __builtin_amdgcn_rcpf(x);
+#else
+        return 1.0f / x;
+#endif
+    }
 };
 template <> struct Fn<double> {
     static ASPH_DH double exp(double x) { return ::exp(x); }
     static ASPH_DH double pow(double x, double y) { return ::pow(x, y); }
     static ASPH_DH double sqrt(double x) { return ::sqrt(x); }
     static ASPH_DH double fma(double a, double b, double c) { return ::fma(a, b, c); }
+    static ASPH_DH double rcp(double x) { return 1.0 / x; }
 };
 
 // ---- one-lane dual number ----------------------------------------------------------------------
@@ -310,6 +320,79 @@ ASPH_DH void mom_vjp(const DevParams& P, R dt, int mode, const R in[MOM_IN], con
     }
 }
 
+// The vector-Jacobian product of mom_stage in reverse mode: the stage's few nonlinear terms are
+// differentiated by hand and its height-dependent reference profiles (theta_ref, qv_ref, the
+// reference pressure) by one dual pass in z, instead of one forward pass per input. Clamps pass
+// the derivative only where mom_stage's comparison selects the argument (the same tests).
+template <class R>
+ASPH_DH void mom_vjp_rev(const DevParams& P, R dt, int mode, const R in[MOM_IN], const R ob[MOM_OUT],
+                         R ib[MOM_IN]) {
+    for (int k = 0; k < MOM_IN; ++k) ib[k] = R(0);
+    const R z = in[9], th = in[10], qv = in[11], qc = in[12], pr = in[13];
+    const Dual<R> zd(z, R(1));
+    const Dual<R> thvr_d = g_theta_ref(P, zd) * (Dual<R>(R(1)) + Dual<R>(R(0.608f)) * g_qv_ref(P, zd));
+    const R thvr = thvr_d.v;
+    const R thv = th * (R(1) + R(0.608f) * qv);
+    R ab[3] = {ob[0], ob[1], ob[2]};   // adjoints of ax, ay, az
+    R thb = R(0), zb = R(0);
+    if (mode == 0) {
+        thb = ob[3];
+        ib[6] = ob[4]; ib[7] = ob[5]; ib[8] = ob[6];
+    } else {
+        const R az = in[2] + R(GRAV) * ((thv - thvr) / thvr - qc);
+        const Dual<R> pz_d = g_pres_of_exner(P, g_exner_ref(P, zd));
+        const R u_raw = pz_d.v + pr;
+        const bool u_on = !(R(1) > u_raw);   // gmax(p + p', 1) selects p + p'
+        const R u = u_on ? u_raw : R(1);
+        const R Pi = Fn<R>::pow(u / R(P0), R(P.rd_cp));
+        const R vdot = in[8] * in[5] + (in[7] * in[4] + in[6] * in[3]);
+        const R dth_raw = -vdot / (R(CP) * Pi);
+        const R cap = R(P.dthdt_cap);
+        const bool d_on = !(-cap > dth_raw) && !(cap < dth_raw);
+        const R dth = d_on ? dth_raw : (-cap > dth_raw ? -cap : cap);
+        const R th2 = th + dth * dt;
+        const bool t_on = !(R(100) > th2) && !(R(P.theta_cap) < th2);
+        const R hdt = R(0.5f) * dt;
+        // velocities (second half kick, sponge on vz)
+        R vzb = ob[6];
+        if ((P.flags & ASPH_F_SPONGE) && z >= R(P.sponge_zs)) {
+            const R tt = (z - R(P.sponge_zs)) / R(P.sponge_depth);
+            const R sg = (R(P.sponge_sigma) * tt) * tt;
+            const R f = R(1) / (R(1) + sg * dt);
+            const R vz0 = in[8] + az * hdt;
+            const R sgb = ob[6] * vz0 * (-(f * f) * dt);
+            zb += sgb * (R(2) * R(P.sponge_sigma) * tt) / R(P.sponge_depth);
+            vzb = ob[6] * f;
+        }
+        ib[6] += ob[4]; ib[7] += ob[5]; ib[8] += vzb;
+        ab[0] += ob[4] * hdt; ab[1] += ob[5] * hdt; ab[2] += vzb * hdt;
+        // theta update
+        const R th2b = t_on ? ob[3] : R(0);
+        thb += th2b;
+        const R dthb = d_on ? th2b * dt : R(0);
+        const R vdotb = -dthb / (R(CP) * Pi);
+        const R Pib = dthb * vdot / ((R(CP) * Pi) * Pi);
+        ib[5] += vdotb * in[8]; ib[8] += vdotb * in[5];
+        ib[4] += vdotb * in[7]; ib[7] += vdotb * in[4];
+        ib[3] += vdotb * in[6]; ib[6] += vdotb * in[3];
+        if (u_on) {
+            const R ub = Pib * (R(P.rd_cp) * Pi / u);
+            ib[13] += ub;
+            zb += ub * pz_d.d;
+        }
+    }
+    // buoyancy of az: GRAV ((thv - thvr) / thvr - qc)
+    ib[0] += ab[0]; ib[1] += ab[1]; ib[2] += ab[2];
+    const R bb = ab[2] * R(GRAV);
+    const R thvb = bb / thvr;
+    zb += (-bb * thv / (thvr * thvr)) * thvr_d.d;
+    ib[12] += -bb;
+    thb += thvb * (R(1) + R(0.608f) * qv);
+    ib[11] += thvb * th * R(0.608f);
+    ib[10] += thb;
+    ib[9] += zb;
+}
+
 // d(rho)/d(S1, S2, z) times rho_bar.
 template <class R> ASPH_DH void dens_vjp(const DevParams& P, R S1, R S2, R z, R rbar, R& s1bar, R& s2bar, R& zbar) {
     const R in[3] = {S1, S2, z};
@@ -340,7 +423,7 @@ ASPH_DH bool dens_pair(const DevParams& P, R rx, R ry, R rz, R& W, R gw[3]) {
     } else {
         const R t = R(2) - q;
         W = (R(0.25f) * t) * (t * t);
-        kq = -(R(0.75f) * t * t) / q;
+        kq = -(R(0.75f) * t * t) * Fn<R>::rcp(q);
     }
     const R f = kq * R(P.inv_h) * R(P.inv_h);
     gw[0] = f * dxm * ix; gw[1] = f * dym * iy; gw[2] = f * dzm * iz;
@@ -413,6 +496,110 @@ ASPH_DH bool mom_pair_vjp(const DevParams& P, R rx, R ry, R rz, const R vo[3], c
         }
         b.rho_o += -Vb * visc / rbar;
         b.rho_n += Vb * (-visc / rbar - visc / rho_n);
+    }
+    return true;
+}
+
+
+// Both orientations of one momentum pair at once, as the reverse gather of particle i needs them:
+// (i, j) of i's sum with i's seeds (Ai, AVi) and (j, i) of j's sum with j's seeds (Aj, AVj),
+// r = x_i - x_j. The geometry (kernel gradient, density and pressure factors, the viscous
+// denominator's common part) is evaluated once; the terms are those of mom_pair_vjp, with the
+// second orientation's r, v_o - v_n and kernel gradient negated. A zero seed drops its
+// orientation (the terms are linear in the seeds). Adds the adjoints of x_i, v_i, rho_i and p_i.
+// False outside the support or for coincident particles.
+template <class R>
+ASPH_DH bool mom_pair_vjp2(const DevParams& P, R rx, R ry, R rz, const R vi[3], const R vj[3], R rho_i, R rho_j,
+                           R p_i, R p_j, const R Ai[3], const R AVi[3], const R Aj[3], const R AVj[3], R xb[3],
+                           R vb[3], R& rb, R& pb) {
+    const R invd[3] = {R(P.inv_dx), R(P.inv_dy), R(P.inv_dz)};
+    const R rv[3] = {rx, ry, rz};
+    const R dt3[3] = {rx * invd[0], ry * invd[1], rz * invd[2]};
+    const R r2t = Fn<R>::fma(dt3[2], dt3[2], Fn<R>::fma(dt3[1], dt3[1], dt3[0] * dt3[0]));
+    if (!(r2t < R(P.lim)) || r2t < R(1e-12f)) return false;
+    const R q = Fn<R>::sqrt(r2t) * R(P.inv_h);
+    const R iq = Fn<R>::rcp(q);
+    R kk, dk;
+    if (q < R(1)) {
+        kk = Fn<R>::fma(R(2.25f), q, R(-3.0f)) * R(P.kin);
+        dk = R(2.25f) * R(P.kin);
+    } else {
+        const R t = R(2) - q;
+        kk = ((R(P.kout) * t) * t) * iq;
+        dk = -(R(P.kout) * t * (q + R(2))) * (iq * iq);
+    }
+    const R c1f = ((dk * R(P.inv_h)) * R(P.inv_h)) * iq;
+    R u[3], g[3];
+    for (int a = 0; a < 3; ++a) {
+        u[a] = dt3[a] * invd[a];
+        g[a] = kk * u[a];
+    }
+    const R inv_i = Fn<R>::rcp(rho_i), inv_j = Fn<R>::rcp(rho_j), inv_ij = inv_i * inv_j;
+    const R dp = (p_j - p_i) * inv_ij;   // (i, j); (j, i) has -dp
+    const R vd[3] = {vi[0] - vj[0], vi[1] - vj[1], vi[2] - vj[2]};
+    const R vdotr = Fn<R>::fma(vd[2], rz, Fn<R>::fma(vd[1], ry, vd[0] * rx));
+    const bool vis = vdotr < R(0);
+    // viscous denominators D = (r^2 + eta^2) rbar^2 rho_n: of (i, j) with rho_j, of (j, i) with rho_i
+    R visc_i = R(0), visc_j = R(0), iDi = R(0), iDj = R(0), irpe = R(0), irbar = R(0);
+    if (vis) {
+        const R rp2 = Fn<R>::fma(rz, rz, Fn<R>::fma(ry, ry, rx * rx));
+        const R rbar = R(0.5f) * (rho_i + rho_j);
+        const R rpe = rp2 + R(P.eta2);
+        irbar = Fn<R>::rcp(rbar);
+        irpe = Fn<R>::rcp(rpe);
+        const R ibase = irpe * (irbar * irbar);
+        iDj = ibase * inv_j;
+        iDi = ibase * inv_i;
+        visc_i = (R(P.visc_c) * vdotr) * iDj;
+        visc_j = (R(P.visc_c) * vdotr) * iDi;
+    }
+    // (i, j): owner i
+    {
+        const R br = dp + visc_i;
+        R G = R(0), Gv = R(0), S = R(0), Gb[3];
+        for (int a = 0; a < 3; ++a) {
+            G += Ai[a] * g[a];
+            Gv += AVi[a] * g[a];
+            Gb[a] = -br * Ai[a] - visc_i * AVi[a];
+            S += Gb[a] * u[a];
+        }
+        for (int a = 0; a < 3; ++a) xb[a] += c1f * S * u[a] + kk * Gb[a] * invd[a] * invd[a];
+        pb += G * inv_ij;
+        rb += G * dp * inv_i;
+        if (vis) {
+            const R Vb = -G - Gv;
+            const R dv = Vb * (R(P.visc_c) * iDj);
+            const R cr = Vb * (-visc_i * irpe) * R(2);
+            for (int a = 0; a < 3; ++a) {
+                vb[a] += dv * rv[a];
+                xb[a] += dv * vd[a] + cr * rv[a];
+            }
+            rb += -Vb * visc_i * irbar;
+        }
+    }
+    // (j, i): owner j, r' = -r, g' = -g, v_o - v_n = -vd; the adjoint of x_i is minus that of r'
+    {
+        const R br = -dp + visc_j;
+        R G = R(0), Gv = R(0), S = R(0), Gb[3];
+        for (int a = 0; a < 3; ++a) {
+            G -= Aj[a] * g[a];
+            Gv -= AVj[a] * g[a];
+            Gb[a] = -br * Aj[a] - visc_j * AVj[a];
+            S += Gb[a] * u[a];   // S' = -S
+        }
+        for (int a = 0; a < 3; ++a) xb[a] -= c1f * S * u[a] + kk * Gb[a] * invd[a] * invd[a];
+        pb -= G * inv_ij;        // p_n of (j, i) is p_i
+        rb += G * (-dp) * inv_i; // rho_n of (j, i) is rho_i
+        if (vis) {
+            const R Vb = -G - Gv;
+            const R dv = Vb * (R(P.visc_c) * iDi);
+            const R cr = Vb * (-visc_j * irpe) * R(2);
+            for (int a = 0; a < 3; ++a) {
+                vb[a] += dv * rv[a];                   // -(dv * r')
+                xb[a] += dv * vd[a] + cr * rv[a];      // -(dv * (-vd) + cr * r')
+            }
+            rb += Vb * (-visc_j * irbar - visc_j * inv_i);
+        }
     }
     return true;
 }

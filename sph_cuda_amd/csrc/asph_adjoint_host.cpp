@@ -1,13 +1,15 @@
 // asph_adjoint_host.cpp - the adjoint driver behind asph_adjoint_rain: forward window with
-// checkpoints, then segment by segment (last first) recomputation with a tape and the reverse
-// sweep of asph_adjoint.hip.
+// checkpoints, then segment by segment (last first) the reverse sweep of asph_adjoint.hip on a
+// tape recorded by recomputing the segment from its checkpoint.
 //
 // Forward schedule: the state is sorted and the forces evaluated (asph_force, asph_h100.cu:913-920),
 // then every step follows the reference loop (asph_h100.cu:962-1038) with the neighbour structure
 // rebuilt at the start of every segment of `checkpoint` steps and kept in between (the stale
 // cell lists of the reference between gated sorts). A checkpoint is the state at a segment
 // start before its sort; recomputing a segment replays the sort, so keys, permutation and cell
-// table are those of the forward.
+// table are those of the forward. The last segment is taped while the forward runs it, so only
+// the others are recomputed. Tape records are batched copies; inside a segment a step's start
+// position, acceleration and density are the previous step's recorded values.
 #include <chrono>
 #include <cstring>
 #include <vector>
@@ -20,13 +22,35 @@ namespace {
 
 struct Alloc {
     std::vector<void*> ptrs;
+    std::vector<hipEvent_t> events;
+    hipStream_t stream = nullptr;
     template <class T> T* get(size_t n) {
         void* p = nullptr;
         if (hipMalloc(&p, std::max<size_t>(n, 1) * sizeof(T)) != hipSuccess) return nullptr;
         ptrs.push_back(p);
         return (T*)p;
     }
-    ~Alloc() { for (void* p : ptrs) (void)hipFree(p); }
+    hipEvent_t event() {
+        hipEvent_t e = nullptr;
+        if (hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) return nullptr;
+        events.push_back(e);
+        return e;
+    }
+    ~Alloc() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+        for (hipEvent_t e : events) (void)hipEventDestroy(e);
+        for (void* p : ptrs) (void)hipFree(p);
+    }
+};
+
+// The neighbour structure of a segment's sort, kept with its tape: the reverse reads it while the
+// handle already sorts for the next recomputation.
+struct SegSort {
+    uint32_t *keys, *cstart, *perm, *nitems;
+    int2* items;
 };
 
 struct Checkpoint {
@@ -41,6 +65,10 @@ struct TapeStep {
     float *r0, *r1, *delta;
     float2* s12;
     int64_t* rain0;
+    // x0, a0 and r0 as used: the previous step's x1, a1 and r1 inside a segment (no sort between)
+    const float4 *x0v = nullptr, *a0v = nullptr;
+    const float* r0v = nullptr;
+    double simt = 0.0;   // simulated time at the start of the step
 };
 
 #define ACHK(expr)                                                                      \
@@ -79,35 +107,77 @@ int restore(asph_t* h, const Checkpoint& c) {
     return ASPH_OK;
 }
 
-// Records the pre-step state into t (x0, v0, a0, t0, r0, rain0).
-int rec_pre(asph_t* h, const TapeStep& t) {
+// The force evaluation at t = 0 (asph_force) with its tape.
+int taped_force(asph_t* h, TapeStep& t, float dt) {
     const size_t N = (size_t)h->N;
     hipStream_t s = h->stream;
-    ACHK(hipMemcpyAsync(t.x0, h->pos, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
-    ACHK(hipMemcpyAsync(t.v0, h->vel, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
-    ACHK(hipMemcpyAsync(t.a0, h->acc, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
-    ACHK(hipMemcpyAsync(t.t0, h->thq, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
-    ACHK(hipMemcpyAsync(t.r0, h->rho, N * sizeof(float), hipMemcpyDeviceToDevice, s));
-    ACHK(hipMemcpyAsync(t.rain0, h->rain, N * sizeof(int64_t), hipMemcpyDeviceToDevice, s));
+    AdjCopies pre;
+    pre.add(t.x0, h->pos, N); pre.add(t.v0, h->vel, N); pre.add(t.a0, h->acc, N); pre.add(t.t0, h->thq, N);
+    pre.add(t.r0, h->rho, N); pre.add(t.rain0, h->rain, N);
+    launch_adj_copies(s, pre);
+    t.x0v = t.x0; t.a0v = t.a0; t.r0v = t.r0;
+    t.simt = 0.0;
+    ACHK(hipMemsetAsync(h->dep, 0, N * sizeof(int64_t), s));
+    h->rec_s12 = t.s12;
+    do_density_eos(h, dt, 0.0, 1, false, true);
+    h->rec_s12 = nullptr;
+    AdjCopies mid;
+    mid.add(t.x1, h->pos, N); mid.add(t.v1, h->vel, N); mid.add(t.r1, h->rho, N); mid.add(t.t1, h->thq, N);
+    launch_adj_copies(s, mid);
+    launch_rain_merge(s, (int)N, h->rain, h->dep);
+    h->dep_pending = false;
+    h->rec_av = t.av;
+    launch_momentum_kdk(h->L, h->S, s, mom_args(h, dt, 0));
+    h->rec_av = nullptr;
+    launch_adj_delta(s, h->disp, h->disp_slots, t.delta);
+    AdjCopies post;
+    post.add(t.a1, h->acc, N);
+    launch_adj_copies(s, post);
+    h->initialized = true;
+    ACHK(hipGetLastError());
     return ASPH_OK;
 }
 
-// Records the values after the density/EOS stage (x1, v1 with p', r1, t1).
-int rec_mid(asph_t* h, const TapeStep& t) {
+// One step of the reference loop (issue_step without a sort) with its tape. prev: the previous
+// step of the same segment, whose x1, a1 and r1 are this step's x0, a0 and r0; null after a sort.
+int taped_step(asph_t* h, TapeStep& t, const TapeStep* prev, float dt) {
     const size_t N = (size_t)h->N;
     hipStream_t s = h->stream;
-    ACHK(hipMemcpyAsync(t.x1, h->pos, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
-    ACHK(hipMemcpyAsync(t.v1, h->vel, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
-    ACHK(hipMemcpyAsync(t.r1, h->rho, N * sizeof(float), hipMemcpyDeviceToDevice, s));
-    ACHK(hipMemcpyAsync(t.t1, h->thq, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
+    AdjCopies pre;
+    pre.add(t.v0, h->vel, N); pre.add(t.t0, h->thq, N); pre.add(t.rain0, h->rain, N);
+    if (prev) {
+        t.x0v = prev->x1; t.a0v = prev->a1; t.r0v = prev->r1;
+    } else {
+        pre.add(t.x0, h->pos, N); pre.add(t.a0, h->acc, N); pre.add(t.r0, h->rho, N);
+        t.x0v = t.x0; t.a0v = t.a0; t.r0v = t.r0;
+    }
+    launch_adj_copies(s, pre);
+    t.simt = h->simt;
+    do_kick_drift(h, dt, 2, true);
+    h->rec_s12 = t.s12;
+    do_density_eos(h, dt, h->simt, 1 | 2, false, true);
+    h->rec_s12 = nullptr;
+    AdjCopies mid;
+    mid.add(t.x1, h->pos, N); mid.add(t.v1, h->vel, N); mid.add(t.r1, h->rho, N); mid.add(t.t1, h->thq, N);
+    launch_adj_copies(s, mid);
+    h->rec_av = t.av;
+    do_momentum_kdk(h, dt, true);
+    h->rec_av = nullptr;
+    launch_adj_delta(s, h->disp, h->disp_slots, t.delta);
+    AdjCopies post;
+    post.add(t.a1, h->acc, N);
+    launch_adj_copies(s, post);
+    do_rain(h, dt, true);
+    h->simt += dt;
+    ACHK(hipGetLastError());
     return ASPH_OK;
 }
 
-AdjTape tape_view(asph_t* h, const TapeStep& t) {
+AdjTape tape_view(const SegSort& g, const TapeStep& t) {
     AdjTape T;
-    T.x0 = t.x0; T.v0 = t.v0; T.a0 = t.a0; T.t0 = t.t0; T.r0 = t.r0; T.rain0 = t.rain0;
+    T.x0 = t.x0v; T.v0 = t.v0; T.a0 = t.a0v; T.t0 = t.t0; T.r0 = t.r0v; T.rain0 = t.rain0;
     T.x1 = t.x1; T.v1 = t.v1; T.r1 = t.r1; T.s12 = t.s12; T.t1 = t.t1; T.a1 = t.a1; T.av = t.av;
-    T.keys = h->keys_sorted; T.cstart = h->cstart; T.delta = t.delta;
+    T.keys = g.keys; T.cstart = g.cstart; T.delta = t.delta;
     return T;
 }
 
@@ -124,6 +194,7 @@ int asph_adjoint_rain(asph_t* h, const asph_adjoint_args* a, float* dJ_dtheta0, 
     const int S = h->S;
     if (cfg.nx < 2 * S + 3 || cfg.ny < 2 * S + 3)
         return fail("the adjoint needs nx, ny >= 2*stencil+3 (got %d, %d)", cfg.nx, cfg.ny);
+    if (h->N >= (int64_t)1 << 30) return fail("the adjoint handles fewer than 2^30 particles");
     const int64_t nsteps = a->nsteps;
     const float dt = a->dt;
     const int ck = a->checkpoint > 0 ? a->checkpoint : 10;
@@ -142,16 +213,32 @@ int asph_adjoint_rain(asph_t* h, const asph_adjoint_args* a, float* dJ_dtheta0, 
         if (!c.pos || !c.vel || !c.acc || !c.thq || !c.rho || !c.rain || !c.precip)
             return fail("adjoint: out of memory (checkpoints)");
     }
-    std::vector<TapeStep> tape((size_t)ck + 1);   // the last slot holds the force evaluation
-    for (auto& t : tape) {
-        t.x0 = A.get<float4>(N); t.v0 = A.get<float4>(N); t.a0 = A.get<float4>(N); t.t0 = A.get<float4>(N);
-        t.x1 = A.get<float4>(N); t.v1 = A.get<float4>(N); t.t1 = A.get<float4>(N); t.a1 = A.get<float4>(N);
-        t.av = A.get<float4>(N); t.r0 = A.get<float>(N); t.r1 = A.get<float>(N); t.s12 = A.get<float2>(N);
-        t.rain0 = A.get<int64_t>(N); t.delta = A.get<float>(4);
-        if (!t.delta || !t.x0 || !t.v0 || !t.a0 || !t.t0 || !t.x1 || !t.v1 || !t.t1 || !t.a1 || !t.av || !t.r0 || !t.r1 ||
-            !t.s12 || !t.rain0)
-            return fail("adjoint: out of memory (tape)");
+    // two tape sets: segment sg uses set sg % 2, so the reverse of one segment (stream s1) runs
+    // while the handle recomputes the one before it (the handle's stream)
+    std::vector<TapeStep> tape[2];
+    SegSort ss[2];
+    const int nitem_max = h->L.nitem_max;
+    for (int b = 0; b < 2; ++b) {
+        tape[b].resize((size_t)ck + 1);   // the last slot holds the force evaluation (set 0)
+        for (auto& t : tape[b]) {
+            t.x0 = A.get<float4>(N); t.v0 = A.get<float4>(N); t.a0 = A.get<float4>(N); t.t0 = A.get<float4>(N);
+            t.x1 = A.get<float4>(N); t.v1 = A.get<float4>(N); t.t1 = A.get<float4>(N); t.a1 = A.get<float4>(N);
+            t.av = A.get<float4>(N); t.r0 = A.get<float>(N); t.r1 = A.get<float>(N); t.s12 = A.get<float2>(N);
+            t.rain0 = A.get<int64_t>(N); t.delta = A.get<float>(4);
+            if (!t.delta || !t.x0 || !t.v0 || !t.a0 || !t.t0 || !t.x1 || !t.v1 || !t.t1 || !t.a1 || !t.av || !t.r0 ||
+                !t.r1 || !t.s12 || !t.rain0)
+                return fail("adjoint: out of memory (tape)");
+        }
+        ss[b].keys = A.get<uint32_t>(N); ss[b].perm = A.get<uint32_t>(N);
+        ss[b].cstart = A.get<uint32_t>((size_t)h->nkeys + 1); ss[b].items = A.get<int2>((size_t)nitem_max);
+        ss[b].nitems = A.get<uint32_t>(1);
+        if (!ss[b].keys || !ss[b].perm || !ss[b].cstart || !ss[b].items || !ss[b].nitems)
+            return fail("adjoint: out of memory (sort)");
     }
+    hipEvent_t e_rec[2] = {A.event(), A.event()}, e_rev[2] = {A.event(), A.event()};
+    if (!e_rec[0] || !e_rec[1] || !e_rev[0] || !e_rev[1]) return fail("adjoint: cannot create events");
+    ACHK(hipStreamCreateWithFlags(&A.stream, hipStreamNonBlocking));   // default priority
+    hipStream_t s1 = A.stream;
     AdjState st[2];
     for (auto& x : st) {
         x.x = A.get<float4>(N); x.v = A.get<float4>(N); x.a = A.get<float4>(N); x.t = A.get<float4>(N);
@@ -166,24 +253,51 @@ int asph_adjoint_rain(asph_t* h, const asph_adjoint_args* a, float* dJ_dtheta0, 
     if (!W.Ab || !W.AVb || !W.x1b || !W.v1b || !W.t2b || !W.pb || !W.r1b || !W.sb || !W.rb_cur || !qc0b)
         return fail("adjoint: out of memory (work)");
 
-    // ---- forward window --------------------------------------------------------------------------
+    // ---- forward window; the last segment is taped as it runs ----------------------------------
     if (reset_aux(h, true) != ASPH_OK) return ASPH_ERR;   // as after asph_set_state
     h->rec_s12 = nullptr;
     h->rec_av = nullptr;
     ACHK(hipStreamSynchronize(s));
     auto t0 = std::chrono::steady_clock::now();
+    TapeStep& tinit = tape[0][(size_t)ck];
+    const int64_t klast = (nseg - 1) * ck;   // first step of the last segment
+    // runs segment sg's steps with the tape (after its sort); the force evaluation first for sg = 0
+    auto taped_segment = [&](int64_t sg) -> int {
+        const int b = (int)(sg & 1);
+        AdjCopies c;
+        c.add(ss[b].keys, h->keys_sorted, N); c.add(ss[b].perm, h->perm, N);
+        c.add(ss[b].cstart, h->cstart, (size_t)h->nkeys + 1); c.add(ss[b].items, h->L.items, (size_t)nitem_max);
+        c.add(ss[b].nitems, h->L.nitems, 1);
+        launch_adj_copies(s, c);
+        if (sg == 0 && taped_force(h, tinit, dt) != ASPH_OK) return ASPH_ERR;
+        std::vector<TapeStep>& tp = tape[b];
+        const int64_t k0 = sg * ck, k1 = std::min<int64_t>(nsteps, k0 + ck);
+        for (int64_t k = k0; k < k1; ++k)
+            if (taped_step(h, tp[(size_t)(k - k0)], k > k0 ? &tp[(size_t)(k - k0 - 1)] : nullptr, dt) != ASPH_OK)
+                return ASPH_ERR;
+        ACHK(hipEventRecord(e_rec[b], s));
+        return ASPH_OK;
+    };
     cps[0].simt = 0.0;
     if (save(h, cps[0]) != ASPH_OK) return ASPH_ERR;
     do_sort(h, true);
-    if (asph_force(h, dt) != ASPH_OK) return ASPH_ERR;
-    for (int64_t k = 0; k < nsteps; ++k) {
-        bool srt = k > 0 && k % ck == 0;
-        if (srt) {
-            cps[(size_t)(k / ck)].simt = h->simt;
-            if (save(h, cps[(size_t)(k / ck)]) != ASPH_OK) return ASPH_ERR;
+    if (nseg == 1) {
+        if (taped_segment(0) != ASPH_OK) return ASPH_ERR;
+    } else {
+        if (asph_force(h, dt) != ASPH_OK) return ASPH_ERR;
+        for (int64_t k = 0; k < klast; ++k) {
+            bool srt = k > 0 && k % ck == 0;
+            if (srt) {
+                cps[(size_t)(k / ck)].simt = h->simt;
+                if (save(h, cps[(size_t)(k / ck)]) != ASPH_OK) return ASPH_ERR;
+            }
+            issue_step(h, dt, h->simt, srt, false, true);
+            h->simt += dt;
         }
-        issue_step(h, dt, h->simt, srt, false, true);
-        h->simt += dt;
+        cps[(size_t)(nseg - 1)].simt = h->simt;
+        if (save(h, cps[(size_t)(nseg - 1)]) != ASPH_OK) return ASPH_ERR;
+        do_sort(h, true);
+        if (taped_segment(nseg - 1) != ASPH_OK) return ASPH_ERR;
     }
     ACHK(hipStreamSynchronize(s));
     const double fwd_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -196,77 +310,56 @@ int asph_adjoint_rain(asph_t* h, const asph_adjoint_args* a, float* dJ_dtheta0, 
         J = (double)(sum * (long double)FX_INV) * (double)Ph.pmass;
     }
 
-    // ---- reverse ---------------------------------------------------------------------------------
+    // ---- reverse: segments last first on stream s1; the segment before is recomputed meanwhile ----
     t0 = std::chrono::steady_clock::now();
     int cur = 0;
-    ACHK(hipMemsetAsync(st[cur].x, 0, N * sizeof(float4), s));
-    ACHK(hipMemsetAsync(st[cur].v, 0, N * sizeof(float4), s));
-    ACHK(hipMemsetAsync(st[cur].a, 0, N * sizeof(float4), s));
-    ACHK(hipMemsetAsync(st[cur].t, 0, N * sizeof(float4), s));
-    ACHK(hipMemsetAsync(st[cur].rho, 0, N * sizeof(float), s));
-    ACHK(hipMemsetAsync(qc0b, 0, sizeof(double), s));
+    ACHK(hipMemsetAsync(st[cur].x, 0, N * sizeof(float4), s1));
+    ACHK(hipMemsetAsync(st[cur].v, 0, N * sizeof(float4), s1));
+    ACHK(hipMemsetAsync(st[cur].a, 0, N * sizeof(float4), s1));
+    ACHK(hipMemsetAsync(st[cur].t, 0, N * sizeof(float4), s1));
+    ACHK(hipMemsetAsync(st[cur].rho, 0, N * sizeof(float), s1));
+    ACHK(hipMemsetAsync(qc0b, 0, sizeof(double), s1));
     {
         std::vector<double> r0(N, (double)Ph.pmass);   // dJ/d(rain in particle masses)
-        ACHK(hipMemcpyAsync(st[cur].rain, r0.data(), N * sizeof(double), hipMemcpyHostToDevice, s));
-        ACHK(hipStreamSynchronize(s));
+        ACHK(hipMemcpyAsync(st[cur].rain, r0.data(), N * sizeof(double), hipMemcpyHostToDevice, s1));
+        ACHK(hipStreamSynchronize(s1));
     }
     const float frac = (cfg.flags & ASPH_F_SEDIMENT) ? rain_frac(Ph, dt) : 0.f;
-    TapeStep& tinit = tape[(size_t)ck];
+    bool rev_issued[2] = {false, false};
     for (int64_t sg = nseg - 1; sg >= 0; --sg) {
-        const Checkpoint& c = cps[(size_t)sg];
-        if (restore(h, c) != ASPH_OK) return ASPH_ERR;
-        do_sort(h, true);
-        if (sg == 0) {   // force evaluation at t = 0, recorded
-            if (rec_pre(h, tinit) != ASPH_OK) return ASPH_ERR;
-            ACHK(hipMemsetAsync(h->dep, 0, N * sizeof(int64_t), s));
-            h->rec_s12 = tinit.s12;
-            do_density_eos(h, dt, 0.0, 1, false, true);
-            h->rec_s12 = nullptr;
-            if (rec_mid(h, tinit) != ASPH_OK) return ASPH_ERR;
-            launch_rain_merge(s, (int)N, h->rain, h->dep);
-            h->dep_pending = false;
-            h->rec_av = tinit.av;
-            launch_momentum_kdk(h->L, h->S, s, mom_args(h, dt, 0));
-            h->rec_av = nullptr;
-            launch_adj_delta(s, h->disp, h->disp_slots, tinit.delta);
-            ACHK(hipMemcpyAsync(tinit.a1, h->acc, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
-            h->initialized = true;
+        if (sg > 0) {   // recompute segment sg - 1 into the other set once its last reader is done
+            const int bp = (int)((sg - 1) & 1);
+            if (rev_issued[bp]) ACHK(hipStreamWaitEvent(s, e_rev[bp], 0));
+            if (restore(h, cps[(size_t)(sg - 1)]) != ASPH_OK) return ASPH_ERR;
+            do_sort(h, true);
+            if (taped_segment(sg - 1) != ASPH_OK) return ASPH_ERR;
         }
+        const int b = (int)(sg & 1);
+        ACHK(hipStreamWaitEvent(s1, e_rec[b], 0));
+        Launch Lr = h->L;   // the segment's work items
+        Lr.items = ss[b].items;
+        Lr.nitems = ss[b].nitems;
         const int64_t k0 = sg * ck, k1 = std::min<int64_t>(nsteps, k0 + ck);
-        std::vector<double> simt((size_t)(k1 - k0));
-        for (int64_t k = k0; k < k1; ++k) {
-            TapeStep& t = tape[(size_t)(k - k0)];
-            simt[(size_t)(k - k0)] = h->simt;
-            if (rec_pre(h, t) != ASPH_OK) return ASPH_ERR;
-            do_kick_drift(h, dt, 2, true);
-            h->rec_s12 = t.s12;
-            do_density_eos(h, dt, h->simt, 1 | 2, false, true);
-            h->rec_s12 = nullptr;
-            if (rec_mid(h, t) != ASPH_OK) return ASPH_ERR;
-            h->rec_av = t.av;
-            do_momentum_kdk(h, dt, true);
-            h->rec_av = nullptr;
-            launch_adj_delta(s, h->disp, h->disp_slots, t.delta);
-            ACHK(hipMemcpyAsync(t.a1, h->acc, N * sizeof(float4), hipMemcpyDeviceToDevice, s));
-            do_rain(h, dt, true);
-            h->simt += dt;
-        }
         for (int64_t k = k1 - 1; k >= k0; --k) {
+            const TapeStep& t = tape[b][(size_t)(k - k0)];
             float Ts = 0.f, qs = 0.f;
-            sfc_params(Ph, simt[(size_t)(k - k0)], Ts, qs);
-            launch_adj_step(Pd, (int64_t)N, s, tape_view(h, tape[(size_t)(k - k0)]), W, st[cur], st[1 - cur], qc0b,
-                            dt, Ts, qs, frac, 1);
+            sfc_params(Ph, t.simt, Ts, qs);
+            launch_adj_step(Lr, S, Pd, (int64_t)N, s1, tape_view(ss[b], t), W, st[cur], st[1 - cur], qc0b, dt, Ts,
+                            qs, frac, 1);
             cur = 1 - cur;
         }
         if (sg == 0) {
-            launch_adj_step(Pd, (int64_t)N, s, tape_view(h, tinit), W, st[cur], st[1 - cur], qc0b, dt, 0.f, 0.f,
-                            frac, 0);
+            launch_adj_step(Lr, S, Pd, (int64_t)N, s1, tape_view(ss[b], tinit), W, st[cur], st[1 - cur], qc0b, dt,
+                            0.f, 0.f, frac, 0);
             cur = 1 - cur;
         }
-        launch_adj_unperm((int64_t)N, s, h->perm, st[cur], st[1 - cur]);
+        launch_adj_unperm((int64_t)N, s1, ss[b].perm, st[cur], st[1 - cur]);
         cur = 1 - cur;
+        ACHK(hipEventRecord(e_rev[b], s1));
+        rev_issued[b] = true;
         ACHK(hipGetLastError());
     }
+    ACHK(hipStreamSynchronize(s1));
     ACHK(hipStreamSynchronize(s));
     const double adj_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
 

@@ -98,10 +98,28 @@ struct AdjWork {
     double* rb_cur;
 };
 // mode 1: a KDK step; mode 0: the force evaluation at t = 0 (no motion, no flux, rain merge).
-void launch_adj_step(const DevParams* P, int64_t N, hipStream_t s, const AdjTape& T, const AdjWork& W,
-                     const AdjState& E, const AdjState& Sst, double* qc0b, float dt, float Ts, float qs_sfc,
-                     float frac, int mode);
+void launch_adj_step(const Launch& L, int S, const DevParams* P, int64_t N, hipStream_t s, const AdjTape& T,
+                     const AdjWork& W, const AdjState& E, const AdjState& Sst, double* qc0b, float dt, float Ts,
+                     float qs_sfc, float frac, int mode);
+// The reverse pair stages (kind 0: momentum, 1: density) on the brick decomposition of the
+// segment's sort (asph_nbr.hip).
+void launch_adj_pairs(const Launch& L, int S, hipStream_t s, int kind, const AdjTape& T, const AdjWork& W,
+                      const AdjState& Sst);
 void launch_adj_delta(hipStream_t s, const float* disp, const uint32_t* slots, float* out);
+// Up to 8 device-to-device copies in one launch (sizes in 4-byte words; 16-byte aligned buffers).
+struct AdjCopies {
+    const uint32_t* src[8];
+    uint32_t* dst[8];
+    uint32_t words[8];
+    int count = 0;
+    template <class T> void add(T* d, const T* s, size_t n) {
+        src[count] = (const uint32_t*)s;
+        dst[count] = (uint32_t*)d;
+        words[count] = (uint32_t)(n * sizeof(T) / 4);
+        ++count;
+    }
+};
+void launch_adj_copies(hipStream_t s, const AdjCopies& C);
 void launch_adj_unperm(int64_t N, hipStream_t s, const uint32_t* perm, const AdjState& in, const AdjState& out);
 
 int particle_block_size();

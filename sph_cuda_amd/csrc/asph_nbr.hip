@@ -34,6 +34,7 @@
 // slots (QK entries): the test loop appends, and when any lane of the wavefront has a full queue
 // all lanes drain theirs in order. Lanes therefore spend the expensive pair body only on
 // in-support pairs, in the reference order.
+#include "asph_adjoint.h"
 #include "asph_kernels.h"
 
 namespace asph {
@@ -870,6 +871,376 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 }
 
 // ---------------------------------------------------------------------------------------------
+// Reverse (adjoint) pair stages on the staged halo.
+//
+// The reverse of particle i's momentum or density sum gathers, for i, the pair terms of (i, j) of
+// i's own sum (own: j listed in the stencil of i's current cell) and of (j, i) of j's sum (rev: i
+// listed in the stencil of j's current cell), each with the forward's support test. Every such
+// j lies within the support of i, so (as in the forward walk) it is listed within
+// floor(1 + 2d + R) cells of i's listed cell, d the drift bound since the sort and R the culling
+// radius; by the stencils it is also listed within S + 1 + floor(d). While d < 1 the forward's
+// halo (margin min(S + 1, floor(1 + 2d + R))) therefore holds every partner: each thread walks the
+// halo cells within the margin of its listed cell, skipping cells whose box [c - d, c + 1 + d]
+// lies outside the support (nearest periodic image), and queues in-support partners with their
+// roles. Otherwise (d >= 1: mode B, by brick map as the forward), and for a halo that cannot be
+// staged, each thread walks the global cell table within S + 1 + floor(d) cells.
+// The pair terms (asph_adjoint.h) and the sets of pairs are those of the forward; the sums are
+// float, in walk order.
+// ---------------------------------------------------------------------------------------------
+namespace adjrev {
+
+__device__ __forceinline__ int fold(int d, int n) {   // (-n, n) -> (-n/2, n/2]
+    if (d < 0) d += n;
+    if (d > n / 2) d -= n;
+    return d;
+}
+__device__ __forceinline__ int wrap1(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
+
+// Whether cell (cx, cy, cz) lies in the stencil of current cell (ix, iy, iz), asph_h100.cu:297-312.
+__device__ __forceinline__ bool in_stencil(const DevParams& P, int ix, int iy, int iz, int cx, int cy, int cz) {
+    const int S = P.S;
+    const bool interior = (iz >= S && iz < P.nz - S);
+    const int z0 = interior ? iz - S : max(iz - S, 0);
+    const int z1 = interior ? iz + S : min(iz + S, P.nz - 1);
+    if (cz < z0 || cz > z1) return false;
+    return abs(fold(cx - ix, P.nx)) <= S && abs(fold(cy - iy, P.ny)) <= S;
+}
+
+// Distance [cells] from a point at offset t of a cell's centre to the cell's box widened by d,
+// over the nearest periodic image when n > 0.
+__device__ __forceinline__ float box_gap(float t, float d, int n) {
+    if (n > 0) {
+        const float hn = 0.5f * (float)n;
+        if (t > hn) t -= (float)n;
+        else if (t < -hn) t += (float)n;
+    }
+    return fmaxf(0.f, fabsf(t) - 0.5f - d);
+}
+
+struct Part {
+    float xc, yc, zc;            // position [cells]
+    int lx, ly, lz, ix, iy, iz;  // listed and current cell
+};
+
+__device__ __forceinline__ Part part(const DevParams& P, float4 x, uint32_t lkey) {
+    Part a;
+    a.xc = x.x * P.inv_dx;
+    a.yc = x.y * P.inv_dy;
+    a.zc = x.z * P.inv_dz;
+    a.lx = (int)compact1by2(lkey);
+    a.ly = (int)compact1by2(lkey >> 1);
+    a.lz = (int)compact1by2(lkey >> 2);
+    cell_trunc(P, x.x, x.y, x.z, a.ix, a.iy, a.iz);
+    return a;
+}
+
+// The support test shared by both orientations (negating r is exact, so both square alike).
+__device__ __forceinline__ bool support(const DevParams& P, float4 xi, float4 xj, float& rx, float& ry, float& rz) {
+    rx = xi.x - xj.x;
+    ry = xi.y - xj.y;
+    rz = xi.z - xj.z;
+    min_image(P, rx, ry);
+    const float dxm = rx * P.inv_dx, dym = ry * P.inv_dy, dzm = rz * P.inv_dz;
+    return __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm)) < P.lim;
+}
+
+__device__ __forceinline__ bool rev_role(const DevParams& P, float4 xj, const Part& a) {
+    int jx, jy, jz;
+    cell_trunc(P, xj.x, xj.y, xj.z, jx, jy, jz);
+    return in_stencil(P, jx, jy, jz, a.lx, a.ly, a.lz);
+}
+
+// Global walk: f(j, own) for every particle listed within S + 1 + floor(d) cells of i's listed
+// cell in a cell whose box meets the support.
+template <class F>
+__device__ __forceinline__ void walk_global(const DevParams& P, uint32_t k0, const uint32_t* __restrict__ cstart,
+                                            const float* dl, const Part& a, F f) {
+    const float limc = P.lim * 1.0001f;
+    const int S = P.S;
+    const int Mx = S + 1 + (int)dl[0], My = S + 1 + (int)dl[1], Mz = S + 1 + (int)dl[2];
+    const int nxs = min(2 * Mx + 1, P.nx), nys = min(2 * My + 1, P.ny);
+    for (int cz = max(a.lz - Mz, 0); cz <= min(a.lz + Mz, P.nz - 1); ++cz) {
+        const float ez = box_gap(a.zc - ((float)cz + 0.5f), dl[2], 0);
+        const float ez2 = ez * ez;
+        if (ez2 >= limc) continue;
+        for (int oy = 0; oy < nys; ++oy) {
+            const int cyu = a.ly - My + oy;
+            const float ey = box_gap(a.yc - ((float)cyu + 0.5f), dl[1], P.ny);
+            const float eyz = __builtin_fmaf(ey, ey, ez2);
+            if (eyz >= limc) continue;
+            const int cy = wrapi(cyu, P.ny);
+            for (int ox = 0; ox < nxs; ++ox) {
+                const int cxu = a.lx - Mx + ox;
+                const float ex = box_gap(a.xc - ((float)cxu + 0.5f), dl[0], P.nx);
+                if (__builtin_fmaf(ex, ex, eyz) >= limc) continue;
+                const int cx = wrapi(cxu, P.nx);
+                const bool own = in_stencil(P, a.ix, a.iy, a.iz, cx, cy, cz);
+                const uint32_t key = k0 | morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz);
+                const uint32_t j0 = cstart[key], j1 = cstart[key + 1];
+                for (uint32_t j = j0; j < j1; ++j) f(j, own);
+            }
+        }
+    }
+}
+
+// Halo walk of slab s: f(slot, own) for the staged particles of the halo cells within the margin
+// of i's listed cell (which lies in the brick) whose box meets the support.
+template <int S, class F>
+__device__ __forceinline__ void walk_halo(const DevParams& P, const BrickGeo& g, const Smem<S, true>& sm, int s,
+                                          const Part& a, F f) {
+    const float limc = P.lim * 1.0001f;
+    const int mx = (g.Hx - BS) / 2, my = (g.Hy - BS) / 2, mz = (g.Hz - BS) / 2;
+    const int hx = a.lx - g.hx0, hy = a.ly - g.hy0, hz = a.lz - g.hz0;
+    const int L2 = g.Hx * g.Hy;
+    const int base = sm.off[sm.slab[s] * L2];
+    const int za = max(hz - mz, (int)sm.slab[s]), zb = min(hz + mz, (int)sm.slab[s + 1] - 1);
+    for (int z = za; z <= zb; ++z) {
+        const int cz = g.hz0 + z;
+        if (cz < 0 || cz >= P.nz) continue;
+        const float ez = box_gap(a.zc - ((float)cz + 0.5f), sm.delta[2], 0);
+        const float ez2 = ez * ez;
+        if (ez2 >= limc) continue;
+        for (int y = hy - my; y <= hy + my; ++y) {
+            const int cy = wrap1(g.hy0 + y, P.ny);
+            const float ey = box_gap(a.yc - ((float)cy + 0.5f), sm.delta[1], P.ny);
+            const float eyz = __builtin_fmaf(ey, ey, ez2);
+            if (eyz >= limc) continue;
+            const int row = (z * g.Hy + y) * g.Hx;
+            for (int x = hx - mx; x <= hx + mx; ++x) {
+                const int cx = wrap1(g.hx0 + x, P.nx);
+                const float ex = box_gap(a.xc - ((float)cx + 0.5f), sm.delta[0], P.nx);
+                if (__builtin_fmaf(ex, ex, eyz) >= limc) continue;
+                const bool own = in_stencil(P, a.ix, a.iy, a.iz, cx, cy, cz);
+                const int p0 = (int)sm.off[row + x] - base, p1 = (int)sm.off[row + x + 1] - base;
+                for (int p = p0; p < p1; ++p) f(p, own);
+            }
+        }
+    }
+}
+
+// The recorded drift bound of the step (with its rounding margin) and mode A (d < 1 per axis).
+template <int S>
+__device__ __forceinline__ void prologue(const float* __restrict__ delta, Smem<S, true>& sm) {
+    if (threadIdx.x == 0) {
+        bool ok = true;
+        for (int c = 0; c < 3; ++c) {
+            sm.delta[c] = delta[c];
+            ok = ok && delta[c] < 1.0f;
+        }
+        sm.modeA = ok ? 1 : 0;
+    }
+    __syncthreads();
+}
+
+constexpr uint16_t QOWN = 0x8000, QREV = 0x4000, QSLOT = 0x3fff;   // LDS queue entry (slot < MAXP)
+
+// ---- momentum --------------------------------------------------------------------------------
+struct MomI {
+    float4 xi;
+    float vi[3], rho_i, rho_raw, p_i, Ai[3], AVi[3];
+    float xb[3], vb[3], rb, pb;
+};
+
+__device__ __forceinline__ MomI mom_load(const AdjTape& T, const AdjWork& W, uint32_t i) {
+    MomI m;
+    m.xi = T.x1[i];
+    const float4 v = T.v1[i], A = W.Ab[i], AV = W.AVb[i];
+    m.vi[0] = v.x; m.vi[1] = v.y; m.vi[2] = v.z;
+    m.rho_raw = T.r1[i];
+    m.rho_i = fmaxf(m.rho_raw, 1e-3f);
+    m.p_i = v.w;
+    m.Ai[0] = A.x; m.Ai[1] = A.y; m.Ai[2] = A.z;
+    m.AVi[0] = AV.x; m.AVi[1] = AV.y; m.AVi[2] = AV.z;
+    for (int c = 0; c < 3; ++c) m.xb[c] = m.vb[c] = 0.f;
+    m.rb = m.pb = 0.f;
+    return m;
+}
+
+__device__ __forceinline__ void mom_partner(const DevParams& P, const AdjTape& T, const AdjWork& W, MomI& m,
+                                            uint32_t j, float4 xj, bool own, bool rev) {
+    float rx = m.xi.x - xj.x, ry = m.xi.y - xj.y, rz = m.xi.z - xj.z;
+    min_image(P, rx, ry);
+    const float4 vj4 = T.v1[j];
+    const float vj[3] = {vj4.x, vj4.y, vj4.z};
+    const float rho_j = fmaxf(T.r1[j], 1e-3f);
+    // a pair outside one role enters with zero seeds for that orientation
+    float4 Aj4 = make_float4(0.f, 0.f, 0.f, 0.f), AVj4 = Aj4;
+    if (rev) { Aj4 = W.Ab[j]; AVj4 = W.AVb[j]; }
+    const float Aj[3] = {Aj4.x, Aj4.y, Aj4.z}, AVj[3] = {AVj4.x, AVj4.y, AVj4.z};
+    const float z3[3] = {0.f, 0.f, 0.f};
+    adj::mom_pair_vjp2<float>(P, rx, ry, rz, m.vi, vj, m.rho_i, rho_j, m.p_i, vj4.w, own ? m.Ai : z3,
+                              own ? m.AVi : z3, Aj, AVj, m.xb, m.vb, m.rb, m.pb);
+}
+
+__device__ __forceinline__ void mom_store(const AdjWork& W, uint32_t i, const MomI& m) {
+    const float4 x1b = W.x1b[i], v1b = W.v1b[i];
+    W.x1b[i] = make_float4(x1b.x + m.xb[0], x1b.y + m.xb[1], x1b.z + m.xb[2], 0.f);
+    W.v1b[i] = make_float4(v1b.x + m.vb[0], v1b.y + m.vb[1], v1b.z + m.vb[2], 0.f);
+    if (m.rho_raw > 1e-3f) W.r1b[i] += m.rb;
+    W.pb[i] += m.pb;
+}
+
+__device__ __forceinline__ void mom_global(const DevParams& P, uint32_t k0, const AdjTape& T, const AdjWork& W,
+                                           const float* dl, uint32_t i) {
+    MomI m = mom_load(T, W, i);
+    const Part a = part(P, m.xi, T.keys[i] - k0);
+    walk_global(P, k0, T.cstart, dl, a, [&](uint32_t j, bool own) {
+        const float4 xj = T.x1[j];
+        float rx, ry, rz;
+        if (!support(P, m.xi, xj, rx, ry, rz)) return;
+        const bool rev = rev_role(P, xj, a);
+        if (own || rev) mom_partner(P, T, W, m, j, xj, own, rev);
+    });
+    mom_store(W, i, m);
+}
+
+// ---- density ---------------------------------------------------------------------------------
+struct DensI {
+    float4 xi;
+    float r0_raw, inv_r0i;
+    float2 sbi;
+    float xb[3], rb;
+};
+
+__device__ __forceinline__ DensI dens_load(const AdjTape& T, const AdjWork& W, uint32_t i) {
+    DensI d;
+    d.xi = T.x1[i];
+    d.r0_raw = T.r0[i];
+    d.inv_r0i = adj::Fn<float>::rcp(fmaxf(d.r0_raw, 1e-3f));
+    d.sbi = W.sb[i];
+    d.xb[0] = d.xb[1] = d.xb[2] = 0.f;
+    d.rb = 0.f;
+    return d;
+}
+
+// W and its gradient are even and odd in r, so one evaluation serves both orientations.
+__device__ __forceinline__ void dens_partner(const DevParams& P, const AdjTape& T, const AdjWork& W, DensI& d,
+                                             uint32_t j, float4 xj, bool own, bool rev) {
+    float rx = d.xi.x - xj.x, ry = d.xi.y - xj.y, rz = d.xi.z - xj.z;
+    min_image(P, rx, ry);
+    float Wv, gw[3];
+    if (!adj::dens_pair<float>(P, rx, ry, rz, Wv, gw)) return;
+    float wb = 0.f;
+    if (own) wb = d.sbi.x + d.sbi.y * adj::Fn<float>::rcp(fmaxf(T.r0[j], 1e-3f));
+    if (rev) {   // (j, i): r_ji = -r_ij, the gradient flips sign and x_i enters with a minus
+        const float2 sbj = W.sb[j];
+        wb += sbj.x + sbj.y * d.inv_r0i;
+        d.rb += sbj.y * Wv * (-d.inv_r0i * d.inv_r0i);
+    }
+    for (int c = 0; c < 3; ++c) d.xb[c] += wb * gw[c];
+}
+
+__device__ __forceinline__ void dens_store(const AdjWork& W, const AdjState& Sst, uint32_t i, const DensI& d) {
+    const float4 x1b = W.x1b[i];
+    W.x1b[i] = make_float4(x1b.x + d.xb[0], x1b.y + d.xb[1], x1b.z + d.xb[2], 0.f);
+    Sst.rho[i] = d.r0_raw > 1e-3f ? d.rb : 0.f;
+}
+
+__device__ __forceinline__ void dens_global(const DevParams& P, uint32_t k0, const AdjTape& T, const AdjWork& W,
+                                            const AdjState& Sst, const float* dl, uint32_t i) {
+    DensI d = dens_load(T, W, i);
+    const Part a = part(P, d.xi, T.keys[i] - k0);
+    walk_global(P, k0, T.cstart, dl, a, [&](uint32_t j, bool own) {
+        const float4 xj = T.x1[j];
+        float rx, ry, rz;
+        if (!support(P, d.xi, xj, rx, ry, rz)) return;
+        const bool rev = rev_role(P, xj, a);
+        if (own || rev) dens_partner(P, T, W, d, j, xj, own, rev);
+    });
+    dens_store(W, Sst, i, d);
+}
+
+}  // namespace adjrev
+
+// One kernel per pair stage, shaped as the forward's (work items of the segment's sort, mode B by
+// brick map). KIND 0: momentum, 1: density. The adjoint runs on one member.
+template <int S, int KIND>
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_adj_pairs_brick(
+        const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
+        const uint32_t* __restrict__ nitems, int nbblk, int csize, AdjTape T, AdjWork W, AdjState Sst) {
+    using namespace adjrev;
+    __shared__ Smem<S, true> sm;
+    const int tid = threadIdx.x;
+    const int b = blockIdx.x;
+    adjrev::prologue<S>(T.delta, sm);
+    const MemberDesc& MD = mem[0];
+    const DevParams& P = MD.P;
+    const uint32_t k0 = MD.k0;
+    auto one_global = [&](uint32_t i) {
+        if (KIND == 0) mom_global(P, k0, T, W, sm.delta, i);
+        else dens_global(P, k0, T, W, Sst, sm.delta, i);
+    };
+    if (!member_mode_a<S, true>(P, sm)) {
+        if (b < nbblk) {
+            const int il = bmap[b].ord * BT + tid;
+            if (il < P.N) one_global((uint32_t)(MD.p0 + il));
+        }
+        return;
+    }
+    if ((uint32_t)b >= *nitems) return;
+    const int2 it = items[b];
+    const uint32_t kb = (uint32_t)bmap[it.x].base;
+    uint32_t o0, o1;
+    item_range(T.cstart, k0, kb, it.y, (uint32_t)csize, o0, o1);
+    if (o0 >= o1) return;
+    const BrickGeo g = brick_geo<S, true>(P, sm, kb);
+    plan<S, true>(P, k0, g, T.cstart, sm);
+    if (sm.nslab == 0) {
+        for (uint32_t i = o0 + tid; i < o1; i += BT) one_global(i);
+        return;
+    }
+    auto rec = [&](Smem<S, true>& s, uint32_t slot, uint32_t jg, float4 pj) {
+        s.p[slot] = make_float4(pj.x, pj.y, pj.z, __uint_as_float(jg));
+    };
+    const int ns = sm.nslab;
+    if (ns == 1) stage_slab<S, true>(P, k0, g, 0, T.x1, T.cstart, sm, rec);
+    for (uint32_t r = o0; r < o1; r += BT) {
+        const uint32_t i = r + tid;
+        const bool active = i < o1;
+        const uint32_t il = active ? i : o0;
+        MomI m;
+        DensI d;
+        if (KIND == 0) m = mom_load(T, W, il);
+        else d = dens_load(T, W, il);
+        const float4 xi = KIND == 0 ? m.xi : d.xi;
+        const Part a = part(P, xi, T.keys[il] - k0);
+        int qn = 0;
+        auto flush = [&]() {
+            for (int k = 0; wave_any(k < qn); ++k) {
+                if (k < qn) {
+                    const uint16_t e = sm.q[k][tid];
+                    const float4 pj = sm.p[e & QSLOT];
+                    const uint32_t j = __float_as_uint(pj.w);
+                    if (KIND == 0) mom_partner(P, T, W, m, j, pj, (e & QOWN) != 0, (e & QREV) != 0);
+                    else dens_partner(P, T, W, d, j, pj, (e & QOWN) != 0, (e & QREV) != 0);
+                }
+            }
+            qn = 0;
+        };
+        for (int s = 0; s < ns; ++s) {
+            if (ns > 1) stage_slab<S, true>(P, k0, g, s, T.x1, T.cstart, sm, rec);
+            if (active) {
+                walk_halo<S>(P, g, sm, s, a, [&](int slot, bool own) {
+                    const float4 pj = sm.p[slot];
+                    float rx, ry, rz;
+                    if (support(P, xi, pj, rx, ry, rz)) {
+                        const bool rev = rev_role(P, pj, a);
+                        if (own || rev)
+                            sm.q[qn++][tid] = (uint16_t)((uint32_t)slot | (own ? QOWN : 0u) | (rev ? QREV : 0u));
+                    }
+                    if (wave_any(qn == QK)) flush();
+                });
+            }
+            flush();
+        }
+        if (active) {
+            if (KIND == 0) mom_store(W, i, m);
+            else dens_store(W, Sst, i, d);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
 // Work items of the brick kernels: every non-empty brick is split into chunks of at most `csize`
 // listed particles, and (order = 1) the chunks are ordered by the brick's estimated cost (chunk
 // size x brick occupancy, 4 bins per octave), heaviest first, so that the long workgroups of dense
@@ -984,6 +1355,21 @@ void launch_momentum_kdk(const Launch& L, int S, hipStream_t s, const MomArgs& A
         case 3: k_momentum_kdk_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, A); break;
         default: k_momentum_kdk_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, A); break;
     }
+}
+
+void launch_adj_pairs(const Launch& L, int S, hipStream_t s, int kind, const AdjTape& T, const AdjWork& W,
+                      const AdjState& Sst) {
+    dim3 g(L.nitem_max);
+#define ASPH_ADJ_PAIRS(SS)                                                                                        \
+    if (kind == 0) k_adj_pairs_brick<SS, 0><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, T, W, Sst); \
+    else k_adj_pairs_brick<SS, 1><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, T, W, Sst);
+    switch (S) {
+        case 1: ASPH_ADJ_PAIRS(1) break;
+        case 2: ASPH_ADJ_PAIRS(2) break;
+        case 3: ASPH_ADJ_PAIRS(3) break;
+        default: ASPH_ADJ_PAIRS(4) break;
+    }
+#undef ASPH_ADJ_PAIRS
 }
 
 }  // namespace asph

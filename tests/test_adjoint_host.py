@@ -106,3 +106,31 @@ def test_gradient_with_stale_lists_beyond_one_cell():
         scale = np.linalg.norm(g[:, 0]) * np.linalg.norm(d)
         print(f"[case] windy dir {k}: FD {fd:.9e} adjoint {ad:.9e}", flush=True)
         assert abs(ad - fd) <= 2e-4 * max(abs(fd), 1e-2 * scale)
+
+
+def test_fused_momentum_pair_matches_two_orientations():
+    """The device's fused evaluation of both orientations of a momentum pair (mom_pair_vjp2)
+    against two evaluations of the single-orientation terms, on random pairs in the support (a
+    quarter with only one orientation seeded): equal up to rounding in double; in float, within
+    the cancellation of the summed terms."""
+    import ctypes as C
+    from oracle.oracle import load
+    cfg, ic, dt = small_config("adj", (12, 12, 10))
+    lib = load()
+    assert lib.oracle_check_mom_pair_fused(C.byref(cfg), 20000, 7, 0) <= 1e-12
+    assert lib.oracle_check_mom_pair_fused(C.byref(cfg), 20000, 7, 1) <= 1e-4
+
+
+@pytest.mark.parametrize("kind", ["adj", "moist"])
+def test_reverse_mode_momentum_epilogue_matches_forward_mode(kind):
+    """The device's reverse-mode product of the momentum epilogue (mom_vjp_rev: analytic terms and
+    one dual pass in z) against the forward-mode product (one dual pass per input), on random
+    inputs that include the clamps of dtheta/dt, of theta and of the pressure and the sponge layer:
+    equal up to rounding in double, within float rounding in float."""
+    import ctypes as C
+    from oracle.oracle import load
+    cfg, ic, dt = small_config(kind, (12, 12, 10))
+    lib = load()
+    for mode in (0, 1):
+        assert lib.oracle_check_mom_vjp_rev(C.byref(cfg), 20000, 11, 0, mode) <= 1e-13
+        assert lib.oracle_check_mom_vjp_rev(C.byref(cfg), 20000, 11, 1, mode) <= 1e-5
