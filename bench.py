@@ -10,7 +10,9 @@ sedimentation, and the per-step CFL/sort-gating readback. dt is held fixed at 0.
 controller monitors only).
 
 Prints one JSON line: particle-steps/s, the HBM-roofline fraction of the dominant kernel from
-HIP events on the library's stream, and the CPU oracle on a bounded sample of the same workload.
+HIP events on the library's stream, the CPU oracle on a bounded sample of the same workload, the
+batched ensemble against sequential runs, and the adjoint/forward time ratio of the 64^3 rain
+adjoint.
 """
 from __future__ import annotations
 
@@ -151,6 +153,54 @@ def ensemble_bench(n_members, steps, seed=5, sequential=True):
     }
 
 
+def adjoint_bench(reps=3, nsteps=60, checkpoint=10):
+    """The adjoint of column-integrated rain at the 64^3 adjoint workload (preset 4): one
+    asph_adjoint_rain call (forward window with checkpoints, recomputation and reverse sweep)
+    against a plain forward of the same window and sort schedule. Best of `reps` of each."""
+    import sph_cuda_amd as s
+
+    cfg, ic, dt, _ = s.preset(4)
+    st = s.ic_generate(cfg, ic)
+    sim = s.Sim(cfg)
+    sim.set_state(st)
+    sim.adjoint_rain(2, dt, checkpoint)   # warm-up (allocations, code paths)
+    best = None
+    for _ in range(reps):
+        r = sim.adjoint_rain(nsteps, dt, checkpoint)
+        if best is None or r["forward_s"] + r["adjoint_s"] < best["forward_s"] + best["adjoint_s"]:
+            best = r
+    plain = None
+    for _ in range(reps):
+        f = s.Sim(cfg)
+        f.set_state(st)
+        f.synchronize()
+        t0 = time.perf_counter()
+        f.sort()
+        f.force(dt)
+        for k in range(nsteps):
+            f.step(dt, k * dt, do_sort=(k > 0 and k % checkpoint == 0))
+        f.synchronize()
+        w = time.perf_counter() - t0
+        plain = w if plain is None else min(plain, w)
+        del f
+    total = best["forward_s"] + best["adjoint_s"]
+    log(f"[bench] adjoint: value+gradient {total * 1e3:.1f} ms, plain forward {plain * 1e3:.1f} ms")
+    return {
+        "workload": f"rain adjoint, {cfg.nx}x{cfg.ny}x{cfg.nz} = {cfg.n} particles, {nsteps} steps of "
+                    f"dt={dt} s, checkpoint every {checkpoint}",
+        "plain_forward_ms": plain * 1e3,
+        "adjoint_call_ms": total * 1e3,
+        "adjoint_forward_pass_ms": best["forward_s"] * 1e3,
+        "adjoint_reverse_ms": best["adjoint_s"] * 1e3,
+        "ratio": (total - plain) / plain,
+        "ratio_definition": "(value+gradient call - plain forward) / plain forward: the reverse "
+                            "sweep, the checkpoint recomputation and the taping, over the forward",
+        "segments": int(best["segments"]),
+        "J_kg": best["J"],
+        "dJ_dqc0": best["dJ_dqc0"],
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -165,6 +215,7 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=240)
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the batched leg of the ensemble (profiling)")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size 3 times (device to device) as "
                          "the byte reference for hardware traffic counters")
@@ -255,10 +306,13 @@ def main():
         },
         "cpu_baseline": None,
         "batched": None,
+        "adjoint": None,
     }
+    del sim
     if not args.no_ensemble:
-        del sim
         out["batched"] = ensemble_bench(args.ensemble_members, args.ensemble_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench()
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
     if not args.no_cpu_baseline:

@@ -288,9 +288,13 @@ int asph_write_dump(asph_t* h, const char* outdir, int32_t sx, int32_t sy, int32
  * reference loop (asph_h100.cu:962-1038) at fixed dt, and rebuilds the neighbour structure at the
  * start of every segment of `checkpoint` steps (the reference's stale cell lists in between). The
  * objective J is the rain water in the domain at the end of the window [kg] (the sum over columns
- * of the column-integrated rain). The reverse sweep recomputes each segment from its checkpoint
- * and differentiates every stage: density, EOS + microphysics, momentum, kick/drift, sedimentation.
- * FP32 state, FP64 reductions (rain grid, threshold derivative). */
+ * of the column-integrated rain). The reverse sweep recomputes each segment but the last from its
+ * checkpoint (on the handle's stream, overlapped with the reverse of the next segment on a
+ * stream of its own) and differentiates every stage: density, EOS + microphysics, momentum,
+ * kick/drift, sedimentation. FP32 state, FP64 reductions (rain grid, threshold derivative), the
+ * EOS stage's derivative in FP64. Allocates for the duration one checkpoint per segment (52 B
+ * per particle), two tapes of checkpoint + 1 steps (about 240 B per particle and step), and the
+ * reverse stream. */
 typedef struct asph_adjoint_args {
     int64_t nsteps;       /* forward window [steps] */
     float dt;             /* fixed step [s] */
@@ -300,7 +304,7 @@ typedef struct asph_adjoint_args {
 typedef struct asph_adjoint_result {
     double J;             /* rain water in the domain at the end of the window [kg] */
     double dJ_dqc0;       /* dJ / d(autoconversion threshold qc_auto) [kg per kg/kg] */
-    double forward_s;     /* wall time of the forward window (with checkpoint copies) */
+    double forward_s;     /* wall time of the forward window (checkpoints, taping of the last segment) */
     double adjoint_s;     /* wall time of the reverse sweep (with the recomputation) */
     int64_t segments;
 } asph_adjoint_result;
