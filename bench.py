@@ -216,6 +216,7 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the batched leg of the ensemble (profiling)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg (profiling)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size 3 times (device to device) as "
                          "the byte reference for hardware traffic counters")
@@ -230,6 +231,9 @@ def main():
     if args.ensemble_only:
         print(json.dumps(ensemble_bench(args.ensemble_members, args.ensemble_steps, sequential=False)),
               flush=True)
+        return
+    if args.adjoint_only:
+        print(json.dumps(adjoint_bench()), flush=True)
         return
     cfg, ic, dt, _ = s.preset(args.preset)
     if args.n:

@@ -278,3 +278,26 @@ def test_large_drift_without_sort():
             x.step(dt, k * dt, do_sort=False)
         bad, errs = state_close(g.get_state(), o.get_state(), TOL)
         assert not bad, (k, errs)
+
+
+def test_water_budget_closes():
+    """With the surface flux off, the water in the particles (vapour + cloud) plus the rain grid and
+    the accumulated precipitation is conserved by the microphysics, sedimentation and transport:
+    the audit's out[3] + out[6] (FP64 sums) stays constant while rain forms."""
+    cfg, ic, dt = small_config("adj", (24, 24, 16))
+    cfg.flags &= ~s._lib.F_SURFACE_FLUX
+    st = s.ic_generate(cfg, ic)
+    g = s.Sim(cfg)
+    g.set_state(st)
+    g.sort()
+    g.force(dt)
+    a0 = g.audit()
+    w0 = a0[3] + a0[6]
+    for k in range(40):
+        g.step(dt, k * dt, do_sort=(k % 10 == 0))
+    a1 = g.audit()
+    w1 = a1[3] + a1[6]
+    print(f"[case] water {w0:.12e} -> {w1:.12e} (rain + precip {a0[6]:.3e} -> {a1[6]:.3e})", flush=True)
+    assert a1[6] > 10.0 * a0[6]          # rain has formed (it is a small part of the total water)
+    # measured 9e-10: the FP32 rounding of the particles' vapour and cloud updates
+    assert abs(w1 - w0) <= 1e-8 * w0
