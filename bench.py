@@ -153,6 +153,19 @@ def ensemble_bench(n_members, steps, seed=5, sequential=True):
     }
 
 
+def valu_probe():
+    """The device's VALU issue roofline (asph_probe_valu): independent non-packed FP32 FMA chains.
+    The neighbour kernels' own instruction counts come from SQ counters (profiles/)."""
+    import ctypes as C
+
+    from sph_cuda_amd import _lib as L
+
+    w = C.c_double()
+    L.check(L.load().asph_probe_valu(8192, 5, C.byref(w)), "asph_probe_valu")
+    return {"wave_fma_per_s": w.value, "fp32_nonpacked_tflops": w.value * 128 / 1e12,
+            "wave_fma_per_simd_cycle_at_2p4GHz": w.value / (2.4e9 * 1024)}
+
+
 def adjoint_bench(reps=3, nsteps=60, checkpoint=10):
     """The adjoint of column-integrated rain at the 64^3 adjoint workload (preset 4): one
     asph_adjoint_rain call (forward window with checkpoints, recomputation and reverse sweep)
@@ -319,6 +332,7 @@ def main():
         out["adjoint"] = adjoint_bench()
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
+    out["valu_probe"] = valu_probe()
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle sample ...")
         out["cpu_baseline"] = cpu_baseline(cfg, st, dt, args.cpu_steps)

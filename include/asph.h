@@ -315,6 +315,14 @@ typedef struct asph_adjoint_result {
 int asph_adjoint_rain(asph_t* h, const asph_adjoint_args* a, float* dJ_dtheta0, float* dJ_dthq4,
                       asph_adjoint_result* r);
 
+/* --- measurement -------------------------------------------------------------------------- */
+
+/* The VALU issue roofline of the current device (no reference counterpart; measurement only):
+ * independent non-packed FP32 FMA chains over 32 wavefronts per CU, `iters` FMAs per chain, the
+ * best of `reps` launches after a warm-up. Writes wave64 FMA instructions per second (times 128
+ * = FLOP/s). Uses the legacy default stream and allocates for the duration. */
+int asph_probe_valu(int32_t iters, int32_t reps, double* wave_fma_per_s);
+
 /* --- tuning of the neighbour kernels ----------------------------------------------------- */
 
 /* Shape of the brick decomposition (no reference counterpart). Each 8x8x8-cell brick's listed

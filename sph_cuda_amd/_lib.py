@@ -176,6 +176,7 @@ SIGNATURES = {
     "asph_get_tuning": (C.c_int, [P, C.POINTER(Tuning)]),
     "asph_autotune": (C.c_int, [P, C.c_int32, C.POINTER(TuneReport)]),
     "asph_get_counters": (C.c_int, [P, C.POINTER(C.c_uint64)]),
+    "asph_probe_valu": (C.c_int, [C.c_int32, C.c_int32, DP]),
 }
 
 _LIB = None

@@ -50,11 +50,26 @@ constexpr int CHUNK_MIN = 256;     // bounds of the work-item size (tuned at run
 constexpr int CHUNK_MAX = 1024;
 constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cells]
 
+// `if (r > h) r -= L; else if (r < -h) r += L;` without branches: the shift is selected and
+// added (an added +0 changes nothing but the sign of a zero difference, which no sum sees).
 __device__ __forceinline__ void min_image(const DevParams& P, float& rx, float& ry) {
     if (P.flags & ASPH_F_MIN_IMAGE) {
-        if (rx > P.hLx) rx -= P.Lx; else if (rx < -P.hLx) rx += P.Lx;
-        if (ry > P.hLy) ry -= P.Ly; else if (ry < -P.hLy) ry += P.Ly;
+        const float sx = (rx > P.hLx ? -P.Lx : 0.f) + (rx < -P.hLx ? P.Lx : 0.f);
+        const float sy = (ry > P.hLy ? -P.Ly : 0.f) + (ry < -P.hLy ? P.Ly : 0.f);
+        rx += sx;
+        ry += sy;
     }
+}
+
+// Correctly rounded square root for 0 and arguments above 2^-96 (squared pair distances): the
+// hardware estimate (within 1 ulp) corrected by the signs of the residuals x - s*s of its two
+// neighbours, the same bits as IEEE sqrtf.
+__device__ __forceinline__ float sqrt_rn(float x) {
+    const float s = __builtin_amdgcn_sqrtf(x);
+    const float sm = __int_as_float(__float_as_int(s) - 1), sp = __int_as_float(__float_as_int(s) + 1);
+    float r = __builtin_fmaf(-sm, s, x) <= 0.f ? sm : s;
+    r = __builtin_fmaf(-sp, s, x) > 0.f ? sp : r;
+    return r;
 }
 
 __device__ __forceinline__ bool wave_any(bool p) { return __ballot(p) != 0ull; }
@@ -92,7 +107,7 @@ __device__ __forceinline__ void dens_sum_global(const DevParams& P, uint32_t k0,
                     float dxm = rx * P.inv_dx, dym = ry * P.inv_dy, dzm = rz * P.inv_dz;
                     float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
                     if (r2 < P.lim) {
-                        float W = fker(sqrtf(r2) * P.inv_h);
+                        float W = fker(sqrt_rn(r2) * P.inv_h);
                         S1 += W;
                         S2 += W * (1.0f / fmaxf(rho_old[j], 1e-3f));
                     }
@@ -110,8 +125,8 @@ struct MomAcc {
 __device__ __forceinline__ void mom_pair(const DevParams& P, float rx, float ry, float rz,
                                          float dxt, float dyt, float dzt, float r2t, float4 vi,
                                          float rhoi, float inv_rhoi, float pri, float rhoj,
-                                         float4 vj, float prj, MomAcc& A) {
-    float q = sqrtf(r2t) * P.inv_h;
+                                         float4 vj, float prj, MomAcc& A, bool act = true) {
+    float q = sqrt_rn(r2t) * P.inv_h;
     float kk = kgrad(P, q);
     float gx = (kk * dxt) * P.inv_dx;
     float gy = (kk * dyt) * P.inv_dy;
@@ -127,12 +142,13 @@ __device__ __forceinline__ void mom_pair(const DevParams& P, float rx, float ry,
         visc = (P.visc_c * vdotr) / (((rp2 + P.eta2) * (rbar * rbar)) * rhoj);
     }
     float br = dp + visc;
-    A.ax = __builtin_fmaf(-br, gx, A.ax);
-    A.ay = __builtin_fmaf(-br, gy, A.ay);
-    A.az = __builtin_fmaf(-br, gz, A.az);
-    A.avx = __builtin_fmaf(-visc, gx, A.avx);
-    A.avy = __builtin_fmaf(-visc, gy, A.avy);
-    A.avz = __builtin_fmaf(-visc, gz, A.avz);
+    // act = false: a placeholder pair of an idle queue lane, evaluated and dropped
+    A.ax = act ? __builtin_fmaf(-br, gx, A.ax) : A.ax;
+    A.ay = act ? __builtin_fmaf(-br, gy, A.ay) : A.ay;
+    A.az = act ? __builtin_fmaf(-br, gz, A.az) : A.az;
+    A.avx = act ? __builtin_fmaf(-visc, gx, A.avx) : A.avx;
+    A.avy = act ? __builtin_fmaf(-visc, gy, A.avy) : A.avy;
+    A.avz = act ? __builtin_fmaf(-visc, gz, A.avz) : A.avz;
 }
 
 template <int S>
@@ -538,7 +554,8 @@ __device__ __forceinline__ int walk_rows(const DevParams& P, const BrickGeo& g, 
             float ey = fmaxf(0.f, fmaxf(((float)jy - delta[1]) - c.yi, c.yi - ((float)(jy + 1) + delta[1])));
             float d2 = __builtin_fmaf(ey, ey, ez2);
             if (d2 >= limc) continue;
-            float ext = sqrtf(limc - d2) + DELTA_EPS;
+            // a culling bound only (the 1e-3-cell margin covers the 1-ulp hardware square root)
+            float ext = __builtin_amdgcn_sqrtf(limc - d2) + DELTA_EPS;
             int xlo = max(max(c.ix - S, g.hx0), (int)ceilf(c.xi - ext - 1.0f - delta[0]));
             int xhi = min(min(c.ix + S, g.hx0 + g.Hx - 1), (int)floorf(c.xi + ext + delta[0]));
             if (xlo > xhi) continue;
@@ -593,16 +610,17 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
             dxm = rx * P.inv_dx; dym = ry * P.inv_dy; dzm = rz * P.inv_dz;
         };
         auto flush = [&]() {
+            // lanes past their queue evaluate a placeholder (slot 0 of the staged slab, which
+            // holds a particle whenever any lane has queued one) and drop it: no divergent region
             for (int k = 0; wave_any(k < qn); ++k) {
-                if (k < qn) {
-                    float4 pj = sm.p[sm.q[k][tid]];
-                    float dxm, dym, dzm;
-                    diff(pj, dxm, dym, dzm);
-                    float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
-                    float W = fker(sqrtf(r2) * P.inv_h);
-                    S1 += W;
-                    S2 += W * pj.w;
-                }
+                const bool act = k < qn;
+                float4 pj = sm.p[act ? sm.q[k][tid] : 0];
+                float dxm, dym, dzm;
+                diff(pj, dxm, dym, dzm);
+                float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+                float W = fker(sqrt_rn(r2) * P.inv_h);
+                S1 = act ? S1 + W : S1;
+                S2 = act ? S2 + W * pj.w : S2;
             }
             npair += qn;
             qn = 0;
@@ -750,18 +768,18 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
             if (WRAP) min_image(P, rx, ry);
         };
         auto flush = [&]() {
+            // lanes past their queue evaluate a placeholder (slot 0, as in the density kernel)
             for (int k = 0; wave_any(k < qn); ++k) {
-                if (k < qn) {
-                    float4 pj = sm.p[sm.q[k][tid]];
-                    const uint32_t jg = __float_as_uint(pj.w);
-                    float4 vj = A.vel[jg];                 // (vx, vy, vz, p'_j)
-                    float rhoj = fmaxf(A.rho[jg], 1e-3f);
-                    float rx, ry, rz;
-                    diff(pj, rx, ry, rz);
-                    float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
-                    float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
-                    mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, rhoj, vj, vj.w, M);
-                }
+                const bool act = k < qn;
+                float4 pj = sm.p[act ? sm.q[k][tid] : 0];
+                const uint32_t jg = __float_as_uint(pj.w);
+                float4 vj = A.vel[jg];                 // (vx, vy, vz, p'_j)
+                float rhoj = fmaxf(A.rho[jg], 1e-3f);
+                float rx, ry, rz;
+                diff(pj, rx, ry, rz);
+                float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
+                float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
+                mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, rhoj, vj, vj.w, M, act);
             }
             npair += qn;
             qn = 0;
