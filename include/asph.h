@@ -246,6 +246,26 @@ int asph_audit(asph_t* h, double out[7]);
  * closed-loop CFL controller (trip counter, velocity clamp, abort -> ASPH_CFL_ABORT), and
  * optional dumps. Continues from the handle's clock (asph_set_state resets it). */
 int asph_run(asph_t* h, const asph_run_args* args, asph_run_stats* stats);
+
+/* asph_run with a caller hook, for drivers that print the reference's reports (asph_h100.cu:
+ * 1121-1159): the hook runs after every dump time (ASPH_EV_DUMP, with the file written, if any)
+ * and every report_every steps (ASPH_EV_REPORT, report_every > 0), with the loop's state. A
+ * nonzero return stops the run (ASPH_OK). With a hook, asph_run's own per-dump lines and audit
+ * are left to the hook; the CFL messages stay (verbose). */
+#define ASPH_EV_DUMP   1
+#define ASPH_EV_REPORT 2
+typedef struct asph_run_event {
+    int32_t kind;
+    int64_t step, sorts, trips, dumps;
+    double simt;          /* simulated time after the step [s] */
+    double wall_s;        /* since the loop started */
+    float dt, dt_required;
+    float umax[3];        /* max |u|, |v|, |w| after the step */
+    const char* dump_path;
+} asph_run_event;
+typedef int (*asph_run_hook)(void* user, asph_t* h, const asph_run_event* ev);
+int asph_run_hooked(asph_t* h, const asph_run_args* args, int32_t report_every, asph_run_hook hook, void* user,
+                    asph_run_stats* stats);
 /* Simulated time and sort-gating clock of the handle. */
 int asph_get_clock(const asph_t* h, double* simt, double* since_sort, float* sort_dt);
 

@@ -92,6 +92,18 @@ class RunArgs(C.Structure):
     ]
 
 
+class RunEvent(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int32), ("step", C.c_int64), ("sorts", C.c_int64), ("trips", C.c_int64),
+        ("dumps", C.c_int64), ("simt", C.c_double), ("wall_s", C.c_double), ("dt", C.c_float),
+        ("dt_required", C.c_float), ("umax", C.c_float * 3), ("dump_path", C.c_char_p),
+    ]
+
+
+EV_DUMP, EV_REPORT = 1, 2
+RUN_HOOK = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_void_p, C.POINTER(RunEvent))
+
+
 class AdjointArgs(C.Structure):
     _fields_ = [("nsteps", C.c_int64), ("dt", C.c_float), ("checkpoint", C.c_int32)]
 
@@ -157,6 +169,7 @@ SIGNATURES = {
     "asph_clamp_vel": (C.c_int, [P, C.c_float]),
     "asph_audit": (C.c_int, [P, DP]),
     "asph_run": (C.c_int, [P, C.POINTER(RunArgs), C.POINTER(RunStats)]),
+    "asph_run_hooked": (C.c_int, [P, C.POINTER(RunArgs), C.c_int32, RUN_HOOK, C.c_void_p, C.POINTER(RunStats)]),
     "asph_get_clock": (C.c_int, [P, DP, DP, FP]),
     "asph_coarse_dump": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, FP]),
     "asph_column_diag": (C.c_int, [P, FP]),

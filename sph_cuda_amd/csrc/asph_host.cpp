@@ -666,7 +666,8 @@ int asph_write_dump(asph_t* h, const char* outdir, int32_t sx, int32_t sy, int32
     return ASPH_OK;
 }
 
-int asph_run(asph_t* h, const asph_run_args* a, asph_run_stats* st) {
+int asph_run_hooked(asph_t* h, const asph_run_args* a, int32_t report_every, asph_run_hook hook, void* user,
+                    asph_run_stats* st) {
     if (!h || !a) return fail("null argument");
     if (h->n_members() > 1 && !a->fixed_dt) return fail("an ensemble runs with a fixed dt");
     const asph_config& c = h->cfgs[0];
@@ -685,6 +686,19 @@ int asph_run(asph_t* h, const asph_run_args* a, asph_run_stats* st) {
     float u[3] = {0, 0, 0};
     int rc = ASPH_OK;
     auto t0 = std::chrono::steady_clock::now();
+    // fills and delivers an event to the hook; nonzero from the hook stops the run
+    auto emit = [&](int32_t kind, const char* path) -> int {
+        if (!hook) return 0;
+        asph_run_event ev;
+        ev.kind = kind;
+        ev.step = step; ev.sorts = sorts; ev.trips = trips; ev.dumps = dumps;
+        ev.simt = h->simt;
+        ev.wall_s = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        ev.dt = dt; ev.dt_required = dt_required;
+        ev.umax[0] = u[0]; ev.umax[1] = u[1]; ev.umax[2] = u[2];
+        ev.dump_path = path;
+        return hook(user, h, &ev);
+    };
     while ((a->sim_end <= 0 || h->simt < a->sim_end) && (a->max_steps <= 0 || step < a->max_steps)) {
         ++step;
         bool sort = (c.flags & ASPH_F_SORT_EVERY_STEP) || h->since_sort >= h->sort_dt;  // :963
@@ -698,11 +712,16 @@ int asph_run(asph_t* h, const asph_run_args* a, asph_run_stats* st) {
         if (!a->fixed_dt) {
             if (adopt < dt_floor) {                                                     // :1054-1071
                 ++trips;
-                if (a->verbose)
-                    fprintf(stderr, "[CFL] step %lld t=%.1fs : required %.4f s < floor %.4f s\n",
-                            (long long)step, h->simt, adopt, dt_floor);
+                if (a->verbose)   // asph_h100.cu:1057-1061
+                    fprintf(stderr,
+                            "[CFL] step %lld t=%.1fs : required %.4f s < floor %.4f s "
+                            "(|u|=%.1f |v|=%.1f |w|=%.1f; binding %s)\n",
+                            (long long)step, h->simt, adopt, dt_floor, u[0], u[1], u[2],
+                            (c.dz / (c.cs + u[2]) <= std::fmin(c.dx / (c.cs + u[0]), c.dy / (c.cs + u[1]))) ? "z" : "xy");
                 if (c.strict_cfl || trips > c.max_trip) {
-                    if (a->verbose) fprintf(stderr, "[CFL] ABORT after %lld violation(s).\n", (long long)trips);
+                    if (a->verbose)
+                        fprintf(stderr, "[CFL] ABORT after %lld violation(s). The scheme is outside its stability "
+                                        "envelope.\n", (long long)trips);
                     rc = ASPH_CFL_ABORT;
                     break;
                 }
@@ -716,16 +735,17 @@ int asph_run(asph_t* h, const asph_run_args* a, asph_run_stats* st) {
         }
         h->sort_dt = sort_interval(c, u, dt);                                           // :1075-1077
         if (h->simt >= next_dump) {                                                     // :1081-1147
+            char path[1024];
+            path[0] = 0;
             if (a->outdir) {
-                char path[1024];
                 if (asph_write_dump(h, a->outdir, a->dump_sx, a->dump_sy, a->dump_sz, h->simt, dt,
                                     path, sizeof(path)) != ASPH_OK)
                     return ASPH_ERR;
                 ++dumps;
-                if (a->verbose) printf("%-8lld +%-8.0fs dt %.4f  CFL %.2f  -> %s\n", (long long)step,
-                                       h->simt, dt, dt_required > 0 ? dt / dt_required : 0.f, path);
+                if (a->verbose && !hook) printf("%-8lld +%-8.0fs dt %.4f  CFL %.2f  -> %s\n", (long long)step,
+                                                h->simt, dt, dt_required > 0 ? dt / dt_required : 0.f, path);
             }
-            if (a->validate) {
+            if (a->validate && !hook) {
                 double tt[7];
                 if (asph_audit(h, tt) != ASPH_OK) return ASPH_ERR;
                 if (a->verbose)
@@ -733,7 +753,9 @@ int asph_run(asph_t* h, const asph_run_args* a, asph_run_stats* st) {
                            tt[0], tt[1], tt[2], tt[0] + tt[1] + tt[2], tt[3], tt[6], tt[4]);
             }
             next_dump += a->dump_every;
+            if (emit(ASPH_EV_DUMP, path[0] ? path : nullptr)) break;
         }
+        if (report_every > 0 && step % report_every == 0 && emit(ASPH_EV_REPORT, nullptr)) break;
     }
     HIPCHK(hipStreamSynchronize(h->stream));
     double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
@@ -744,6 +766,10 @@ int asph_run(asph_t* h, const asph_run_args* a, asph_run_stats* st) {
         st->wall_s = wall;
     }
     return rc;
+}
+
+int asph_run(asph_t* h, const asph_run_args* a, asph_run_stats* st) {
+    return asph_run_hooked(h, a, 0, nullptr, nullptr, st);
 }
 
 int asph_run_graph(asph_t* h, int64_t nsteps, float dt, int32_t sort_every, asph_run_stats* st) {

@@ -225,8 +225,10 @@ class Sim:
 
     def run(self, sim_end: float = 0.0, max_steps: int = 0, dt: float = 0.0, fixed_dt: bool = False,
             dump_every: float = 0.0, dump_stride=(10, 10, 3), outdir: str | None = None,
-            validate: bool = False, verbose: bool = False):
-        """The reference's main loop (asph_h100.cu:958-1078). Returns (rc, RunStats)."""
+            validate: bool = False, verbose: bool = False, hook=None, report_every: int = 0):
+        """The reference's main loop (asph_h100.cu:958-1078). Returns (rc, RunStats).
+        hook(event) (optional) is called with an L.RunEvent after every dump time and every
+        report_every steps (asph_run_hooked); a true return value stops the run."""
         a = L.RunArgs()
         a.sim_end, a.max_steps, a.dt, a.fixed_dt = sim_end, max_steps, dt, int(fixed_dt)
         a.dump_every = dump_every
@@ -234,7 +236,11 @@ class Sim:
         a.outdir = outdir.encode() if outdir else None
         a.validate, a.verbose = int(validate), int(verbose)
         st = L.RunStats()
-        rc = self.lib.asph_run(self.h, C.byref(a), C.byref(st))
+        if hook is None:
+            rc = self.lib.asph_run(self.h, C.byref(a), C.byref(st))
+        else:
+            cb = L.RUN_HOOK(lambda user, h, ev: 1 if hook(ev.contents) else 0)
+            rc = self.lib.asph_run_hooked(self.h, C.byref(a), report_every, cb, None, C.byref(st))
         if rc not in (L.OK, L.CFL_ABORT):
             L.check(rc, "asph_run")
         return rc, st
