@@ -166,6 +166,50 @@ def valu_probe():
             "wave_fma_per_simd_cycle_at_2p4GHz": w.value / (2.4e9 * 1024)}
 
 
+def device_loop_bench(steps=120, poll=32, reps=3):
+    """The reference loop at the 32^3 dry workload (preset 1, CFL-controlled dt): the host loop
+    (asph_run, one host synchronisation per step for max|v|) against the same loop with its
+    controller on the device (asph_run_device, a captured graph and a poll every `poll` steps).
+    Best of `reps`; the final states must agree bit for bit."""
+    import numpy as np
+
+    import sph_cuda_amd as s
+
+    cfg, ic, dt, _ = s.preset(1)
+    st = s.ic_generate(cfg, ic)
+    walls = {"host": None, "device": None}
+    finals = {}
+    for name in ("host", "device"):
+        for _ in range(reps):
+            with s.Sim(cfg) as g:
+                g.set_state(st)
+                g.sort()
+                g.force(dt)
+                g.synchronize()
+                t0 = time.perf_counter()
+                if name == "host":
+                    rc, stats = g.run(max_steps=steps)
+                else:
+                    rc, stats = g.run_device(max_steps=steps, poll=poll)
+                g.synchronize()
+                w = time.perf_counter() - t0
+                if rc != 0:
+                    raise SystemExit(f"device-loop bench: {name} run aborted")
+                walls[name] = w if walls[name] is None else min(walls[name], w)
+                finals[name] = g.get_state()
+    same = all(np.array_equal(getattr(finals["host"], f), getattr(finals["device"], f))
+               for f in ("pos", "vel", "thq", "rho"))
+    n = cfg.n
+    log(f"[bench] device loop: host {walls['host'] * 1e3:.1f} ms, device {walls['device'] * 1e3:.1f} ms, same {same}")
+    return {
+        "workload": f"dry bubble {cfg.nx}x{cfg.ny}x{cfg.nz} = {n} particles, {steps} CFL-controlled steps",
+        "host_loop_value": n * steps / walls["host"], "device_loop_value": n * steps / walls["device"],
+        "unit": "particle-steps/s", "host_loop_ms_per_step": walls["host"] / steps * 1e3,
+        "device_loop_ms_per_step": walls["device"] / steps * 1e3, "speedup": walls["host"] / walls["device"],
+        "poll_steps": poll, "bit_identical": bool(same),
+    }
+
+
 def adjoint_bench(reps=3, nsteps=60, checkpoint=10):
     """The adjoint of column-integrated rain at the 64^3 adjoint workload (preset 4): one
     asph_adjoint_rain call (forward window with checkpoints, recomputation and reverse sweep)
@@ -229,6 +273,7 @@ def main():
     ap.add_argument("--ensemble-only", action="store_true",
                     help="run only the batched leg of the ensemble (profiling)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
+    ap.add_argument("--no-device-loop", action="store_true", help="skip the device-controller leg")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg (profiling)")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size 3 times (device to device) as "
@@ -324,12 +369,15 @@ def main():
         "cpu_baseline": None,
         "batched": None,
         "adjoint": None,
+        "device_loop": None,
     }
     del sim
     if not args.no_ensemble:
         out["batched"] = ensemble_bench(args.ensemble_members, args.ensemble_steps)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench()
+    if not args.no_device_loop:
+        out["device_loop"] = device_loop_bench()
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
     out["valu_probe"] = valu_probe()

@@ -247,6 +247,16 @@ int asph_audit(asph_t* h, double out[7]);
  * optional dumps. Continues from the handle's clock (asph_set_state resets it). */
 int asph_run(asph_t* h, const asph_run_args* args, asph_run_stats* stats);
 
+/* asph_run with the loop's controller on the device (no reference counterpart; the reference
+ * loop of asph_h100.cu:958-1081 without its per-step host synchronisation, :1046): each step's
+ * sort decision, CFL rules (dt, trips, velocity clamp, abort) and sort interval run in a
+ * one-wave kernel inside a captured two-step graph, and the host reads the loop state every
+ * `poll` steps (rounded up to even). Single-domain handles; results and statistics are those of
+ * asph_run bit for bit. Dumps and audits are written at the steps asph_run writes them (the device
+ * pauses until the host has written them); CFL trip messages are not printed, an abort is
+ * (verbose). */
+int asph_run_device(asph_t* h, const asph_run_args* args, int32_t poll, asph_run_stats* stats);
+
 /* asph_run with a caller hook, for drivers that print the reference's reports (asph_h100.cu:
  * 1121-1159): the hook runs after every dump time (ASPH_EV_DUMP, with the file written, if any)
  * and every report_every steps (ASPH_EV_REPORT, report_every > 0), with the loop's state. A

@@ -170,6 +170,7 @@ SIGNATURES = {
     "asph_audit": (C.c_int, [P, DP]),
     "asph_run": (C.c_int, [P, C.POINTER(RunArgs), C.POINTER(RunStats)]),
     "asph_run_hooked": (C.c_int, [P, C.POINTER(RunArgs), C.c_int32, RUN_HOOK, C.c_void_p, C.POINTER(RunStats)]),
+    "asph_run_device": (C.c_int, [P, C.POINTER(RunArgs), C.c_int32, C.POINTER(RunStats)]),
     "asph_get_clock": (C.c_int, [P, DP, DP, FP]),
     "asph_coarse_dump": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, FP]),
     "asph_column_diag": (C.c_int, [P, FP]),

@@ -37,6 +37,7 @@ struct asph_t {
     uint32_t *bh = nullptr, *bo = nullptr, *dtot = nullptr, *cstart = nullptr;
     uint32_t *keys_sorted = nullptr, *perm = nullptr;
     uint32_t* maxv = nullptr;        // [n_members][3][64]
+    asph::StepCtl* ctl_d = nullptr;  // device step controller (asph_run_device)
     float* disp = nullptr;           // drift since the last sort [cells per axis]
     uint32_t* disp_slots = nullptr;  // per-step drift maxima (64 slots per axis)
     double* clock = nullptr;         // device simulated time (graph replay)

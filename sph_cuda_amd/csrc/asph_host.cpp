@@ -173,6 +173,7 @@ MomArgs mom_args(asph_t* h, float dt, int mode) {
     A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
     A.stats = h->stats;
     A.av_out = h->rec_av;
+    A.ctl = h->L.ctl;
     A.dt = dt; A.mode = mode;
     return A;
 }
@@ -183,13 +184,13 @@ void do_sort(asph_t* h, bool launch) {
         ProfScope ps(h, ASPH_K_SORT);
         launch_keys(h->L, h->stream, h->pos, h->keys0);
         launch_radix_sort(h->stream, (int)h->N, h->end_bit, h->keys0, h->kA, h->vA, h->kB, h->vB,
-                          h->bh, h->bo, h->dtot, &h->keys_sorted, &h->perm);
+                          h->bh, h->bo, h->dtot, &h->keys_sorted, &h->perm, h->L.ctl);
         const bool ga = (h->cfgs[0].flags & ASPH_F_GATHER_ACC) != 0;
         launch_gather(h->stream, (int)h->N, h->perm, h->pos, h->pos2, h->vel, h->vel2, h->thq,
-                      h->thq2, h->rho, h->rho2, ga ? h->acc : nullptr, ga ? h->acc2 : nullptr);
+                      h->thq2, h->rho, h->rho2, ga ? h->acc : nullptr, ga ? h->acc2 : nullptr, h->L.ctl);
         launch_cell_start(h->L, h->stream, (int)h->N, h->nkeys, h->keys_sorted, h->cstart);
         launch_brick_items(h->L, h->stream, h->cstart);
-        launch_step_begin(h->stream, h->disp, h->disp_slots, 1, h->clock, 2, 0.0, 0.f);
+        launch_step_begin(h->stream, h->disp, h->disp_slots, 1, h->clock, 2, 0.0, 0.f, h->L.ctl, 1);
     } else {
         // the sorted keys and permutation land in A or B depending on the pass count
         const int npass = (h->end_bit + 7) / 8;
@@ -205,7 +206,7 @@ void do_sort(asph_t* h, bool launch) {
 void do_kick_drift(asph_t* h, float dt, int clock_mode, bool launch) {
     if (launch) {
         ProfScope ps(h, ASPH_K_KICK_DRIFT);
-        launch_step_begin(h->stream, h->disp, h->disp_slots, 0, h->clock, clock_mode, 0.0, dt);
+        launch_step_begin(h->stream, h->disp, h->disp_slots, 0, h->clock, clock_mode, 0.0, dt, h->L.ctl, 0);
         launch_kick_drift(h->L, h->stream, h->pos, h->vel, h->acc, h->dep, h->disp_slots, dt);
     }
     h->dep_pending = false;
@@ -218,6 +219,8 @@ void do_density_eos(asph_t* h, float dt, double simt, int bits, bool use_clock, 
         A.vel = h->vel; A.cstart = h->cstart; A.keys_sorted = h->keys_sorted; A.rain = h->rain;
         A.dep = h->dep; A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
         A.clock = use_clock ? h->clock : nullptr; A.simt = simt;
+        if (h->L.ctl) A.clock = &h->L.ctl->simt;   // device controller: the step's start time
+        A.ctl = h->L.ctl;
         A.stats = h->stats;
         A.s12_out = h->rec_s12;
         A.dt = dt; A.bits = bits;
@@ -772,6 +775,128 @@ int asph_run(asph_t* h, const asph_run_args* a, asph_run_stats* st) {
     return asph_run_hooked(h, a, 0, nullptr, nullptr, st);
 }
 
+int asph_run_device(asph_t* h, const asph_run_args* a, int32_t poll, asph_run_stats* st) {
+    if (!h || !a) return fail("null argument");
+    if (h->n_members() > 1) return fail("the device controller runs single-domain handles");
+    if (a->sim_end <= 0 && a->max_steps <= 0) return fail("asph_run_device needs sim_end or max_steps");
+    const asph_config& c = h->cfgs[0];
+    const float D = std::fmin(std::fmin(c.dx, c.dy), c.dz);
+    const float dt_static = c.cfl * D / c.cs;                                          // :794
+    const float dt_cap = c.dt_cap > 0.f ? std::fmin(c.dt_cap, dt_static) : dt_static;  // :795
+    const float dt_floor = 0.05f * dt_cap;                                             // :796
+    const float dt0 = a->dt > 0.f ? a->dt : dt_cap;
+    if (!h->initialized) {
+        if (asph_sort(h) != ASPH_OK) return ASPH_ERR;
+        if (asph_force(h, dt0) != ASPH_OK) return ASPH_ERR;
+    }
+    const int pairs = std::max(1, (poll + 1) / 2);   // replays of the two-step graph between polls
+    if (!h->ctl_d && h->alloc(&h->ctl_d, 1) != 0) return fail("out of device memory (controller)");
+    StepCtl k;
+    std::memset(&k, 0, sizeof(k));
+    k.simt = h->simt; k.since_sort = h->since_sort; k.sim_end = a->sim_end;
+    k.dump_every = a->dump_every > 0 ? a->dump_every : 0.0;
+    k.next_dump = k.dump_every > 0 ? (std::floor(h->simt / k.dump_every) + 1) * k.dump_every : 0.0;
+    k.max_steps = a->max_steps; k.n = h->N;
+    k.dt = dt0; k.dt_required = dt0; k.sort_dt = h->sort_dt;
+    k.clamp_dt = std::fmax(dt_floor, c.cfl * D / (1.5f * c.cs) * 0.90f);               // :1068-1069
+    k.dt_cap = dt_cap; k.dt_floor = dt_floor; k.cfl = c.cfl; k.cs = c.cs; k.dx = c.dx; k.dy = c.dy; k.dz = c.dz;
+    k.max_trip = c.max_trip; k.strict = c.strict_cfl; k.every = (c.flags & ASPH_F_SORT_EVERY_STEP) ? 1 : 0;
+    k.fixed = a->fixed_dt ? 1 : 0;
+    HIPCHK(hipMemcpyAsync(h->ctl_d, &k, sizeof(k), hipMemcpyHostToDevice, h->stream));
+    h->L.ctl = h->ctl_d;
+    auto t0 = std::chrono::steady_clock::now();
+    int rc = ASPH_OK;
+    int64_t dumps = 0;
+    // one step: controller, sort stages (gated), the four stages, controller, clamp (gated)
+    auto device_step = [&](bool launch) {
+        if (launch) launch_ctl_pre(h->stream, h->ctl_d);
+        do_sort(h, launch);
+        do_kick_drift(h, dt0, 2, launch);
+        do_density_eos(h, dt0, 0.0, 1 | 2, true, launch);
+        do_momentum_kdk(h, dt0, launch);
+        do_rain(h, dt0, launch);
+        if (launch) {
+            launch_ctl_post(h->stream, h->ctl_d, h->maxv);
+            launch_clamp_vel_ctl(h->L, h->stream, h->vel);
+        }
+    };
+    for (;;) {
+        for (int r = 0; r < pairs; ++r) {
+            // a captured pair of steps returns the rotated buffers to their arrangement
+            auto key = std::make_tuple((void*)h->pos, (void*)h->vel, (void*)h->thq, (void*)h->rho,
+                                       (void*)h->acc, (void*)h->rain, 3, 0.f);
+            auto it = h->graphs.find(key);
+            if (it == h->graphs.end()) {
+                hipGraph_t g = nullptr;
+                hipGraphExec_t ge = nullptr;
+                g_capturing = true;
+                HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+                device_step(true);
+                device_step(true);
+                hipError_t e = hipStreamEndCapture(h->stream, &g);
+                g_capturing = false;
+                if (e != hipSuccess) { h->L.ctl = nullptr; return fail("graph capture: %s", hipGetErrorString(e)); }
+                HIPCHK(hipGraphInstantiate(&ge, g, nullptr, nullptr, 0));
+                (void)hipGraphDestroy(g);
+                h->graphs[key] = ge;
+                HIPCHK(hipGraphLaunch(ge, h->stream));
+            } else {
+                HIPCHK(hipGraphLaunch(it->second, h->stream));
+                device_step(false);
+                device_step(false);
+            }
+        }
+        HIPCHK(hipMemcpyAsync(&k, h->ctl_d, sizeof(k), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        if (k.abort) {
+            if (a->verbose)
+                fprintf(stderr, "[CFL] ABORT after %lld violation(s) at step %lld. The scheme is outside its "
+                                "stability envelope.\n", (long long)k.trips, (long long)k.steps);
+            rc = ASPH_CFL_ABORT;
+            break;
+        }
+        if (k.pause) {   // a dump time was reached: the state is that of the step that reached it
+            h->simt = k.simt;
+            if (a->outdir) {
+                char path[1024];
+                if (asph_write_dump(h, a->outdir, a->dump_sx, a->dump_sy, a->dump_sz, k.simt, k.dt, path,
+                                    sizeof(path)) != ASPH_OK) {
+                    h->L.ctl = nullptr;
+                    return ASPH_ERR;
+                }
+                ++dumps;
+                if (a->verbose) printf("%-8lld +%-8.0fs dt %.4f  CFL %.2f  -> %s\n", (long long)k.steps, k.simt,
+                                       k.dt, k.dt_required > 0 ? k.dt / k.dt_required : 0.f, path);
+            }
+            if (a->validate) {
+                double tt[7];
+                if (asph_audit(h, tt) != ASPH_OK) { h->L.ctl = nullptr; return ASPH_ERR; }
+                if (a->verbose)
+                    printf("  AUDIT  KE %.4e  PE %.4e  IE %.4e  E_tot %.4e  water %.4e  rain %.4e  pz %.4e\n",
+                           tt[0], tt[1], tt[2], tt[0] + tt[1] + tt[2], tt[3], tt[6], tt[4]);
+            }
+            k.pause = 0;
+            k.next_dump += k.dump_every;
+            HIPCHK(hipMemcpyAsync(h->ctl_d, &k, sizeof(k), hipMemcpyHostToDevice, h->stream));
+            continue;
+        }
+        if (k.idle) break;   // the run's end was reached
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    const double wall = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    h->L.ctl = nullptr;
+    h->simt = k.simt;
+    h->since_sort = k.since_sort;
+    h->sort_dt = k.sort_dt;
+    if (st) {
+        st->steps = k.steps; st->sorts = k.sorts; st->trips = k.trips; st->dumps = dumps;
+        st->simt = k.simt; st->dt = k.dt; st->dt_required = k.dt_required;
+        st->umax[0] = k.u[0]; st->umax[1] = k.u[1]; st->umax[2] = k.u[2];
+        st->wall_s = wall;
+    }
+    return rc;
+}
+
 int asph_run_graph(asph_t* h, int64_t nsteps, float dt, int32_t sort_every, asph_run_stats* st) {
     if (!h) return fail("null handle");
     if (nsteps < 0 || !(dt > 0.f)) return fail("bad steps or dt");
@@ -851,6 +976,8 @@ int asph_kernel_times(asph_t* h, double ms[ASPH_K_COUNT], int64_t counts[ASPH_K_
 }
 
 // --- tuning of the brick decomposition --------------------------------------------------------
+}  // extern "C"
+
 namespace {
 
 void drop_graphs(asph_t* h) {
@@ -872,6 +999,8 @@ DensArgs trial_dens_args(asph_t* h, float dt) {
 }
 
 }  // namespace
+
+extern "C" {
 
 int asph_set_tuning(asph_t* h, const asph_tuning* t) {
     if (!h || !t) return fail("null argument");

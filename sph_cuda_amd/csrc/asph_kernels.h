@@ -6,6 +6,23 @@
 
 namespace asph {
 
+// Device-resident step controller (asph_run_device): the loop state that asph_run keeps on the
+// host, read and advanced on the device inside a captured graph. Kernels given one take dt from
+// it and do nothing while it is idle (stopped, paused for a dump, or aborted); the sort stages
+// run only on a sort step, the gather then copying without permuting so that the buffer rotation
+// captured in the graph stays the same.
+struct StepCtl {
+    double simt, since_sort;          // start of the current step; time since the last sort
+    double sim_end, next_dump, dump_every;
+    int64_t steps, sorts, max_steps;
+    int64_t n;                        // particles (idle steps copy the rotated buffers)
+    float dt, dt_required, sort_dt, clamp_dt;   // clamp_dt: dt adopted after a velocity clamp
+    float u[3];
+    int32_t do_sort, idle, clamp, trips, abort, pause;
+    float dt_cap, dt_floor, cfl, cs, dx, dy, dz;
+    int32_t max_trip, strict, every, fixed;
+};
+
 // Device tables describing one handle's members and the work maps of its launches.
 struct Launch {
     const MemberDesc* mem;   // [n_members]
@@ -26,6 +43,7 @@ struct Launch {
     uint4* gaps;             // [gapcap]: (first key, last key, value)
     uint32_t* ngaps;         // reset by the keys kernel of each sort
     uint32_t gapcap;
+    const StepCtl* ctl = nullptr;   // device controller of the launches (asph_run_device), or null
 };
 
 struct DensArgs {
@@ -46,6 +64,7 @@ struct DensArgs {
     double simt;                 // then this value is used
     unsigned long long* stats;   // work counters (ASPH_C_DENS_*), or null
     float2* s12_out;             // adjoint tape: the density sums (S1, S2) per particle, or null
+    const StepCtl* ctl = nullptr;   // device controller (dt, idle), or null
     float dt;
     int bits;                    // bit 0: EOS + microphysics, bit 1: surface flux (if flagged)
 };
@@ -65,6 +84,7 @@ struct MomArgs {
     const uint32_t* disp_slots;
     unsigned long long* stats;   // work counters (ASPH_C_MOM_*), or null
     float4* av_out;              // adjoint tape: the viscous acceleration sums (x mm), or null
+    const StepCtl* ctl = nullptr;   // device controller (dt, idle), or null
     float dt;
     int mode;                    // 0: force evaluation, 1: step
 };
@@ -124,7 +144,12 @@ void launch_adj_unperm(int64_t N, hipStream_t s, const uint32_t* perm, const Adj
 
 int particle_block_size();
 void launch_step_begin(hipStream_t s, float* disp, uint32_t* disp_slots, int reset, double* clock,
-                       int clock_mode, double t, float dt);
+                       int clock_mode, double t, float dt, const StepCtl* ctl = nullptr, int sort_gate = 0);
+// The controller's steps: before a step (stop conditions, the sort decision) and after it (time,
+// CFL rules, velocity clamp flag, abort, sort interval, dump pause; asph_h100.cu:1038-1081).
+void launch_ctl_pre(hipStream_t s, StepCtl* ctl);
+void launch_ctl_post(hipStream_t s, StepCtl* ctl, const uint32_t* maxv);
+void launch_clamp_vel_ctl(const Launch& L, hipStream_t s, float4* vel);
 void launch_kick_drift(const Launch& L, hipStream_t s, float4* pos, float4* vel, const float4* acc,
                        int64_t* dep, uint32_t* disp_slots, float dt);
 void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A);
@@ -141,7 +166,7 @@ void launch_rain_merge(hipStream_t s, int n, int64_t* rain, int64_t* dep);
 void launch_keys(const Launch& L, hipStream_t s, const float4* pos, uint32_t* keys);
 void launch_gather(hipStream_t s, int N, const uint32_t* perm, const float4* pA, float4* pB,
                    const float4* vA, float4* vB, const float4* tA, float4* tB, const float* rA,
-                   float* rB, const float4* aA, float4* aB);
+                   float* rB, const float4* aA, float4* aB, const StepCtl* ctl = nullptr);
 uint32_t cell_gap_capacity(uint32_t nkeys);
 void launch_cell_start(const Launch& L, hipStream_t s, int N, uint32_t nkeys, const uint32_t* keys,
                        uint32_t* cstart);
@@ -161,6 +186,6 @@ int radix_tile();
 // On return *keys_sorted and *perm point into kA/vA or kB/vB.
 void launch_radix_sort(hipStream_t s, int N, int end_bit, const uint32_t* keys0, uint32_t* kA,
                        uint32_t* vA, uint32_t* kB, uint32_t* vB, uint32_t* bh, uint32_t* bo,
-                       uint32_t* dtot, uint32_t** keys_sorted, uint32_t** perm);
+                       uint32_t* dtot, uint32_t** keys_sorted, uint32_t** perm, const StepCtl* ctl = nullptr);
 
 }  // namespace asph

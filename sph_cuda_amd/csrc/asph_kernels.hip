@@ -37,8 +37,13 @@ __global__ __launch_bounds__(TPB) void k_kick_drift(const MemberDesc* __restrict
                                                     const float4* __restrict__ acc,
                                                     int64_t* __restrict__ dep,
                                                     uint32_t* __restrict__ disp_slots, float hdt,
-                                                    float dt) {
+                                                    float dt, const StepCtl* __restrict__ ctl) {
     __shared__ float red[3][TPB / 64];
+    if (ctl) {
+        if (ctl->idle) return;
+        dt = ctl->dt;
+        hdt = 0.5f * dt;
+    }
     float mx = 0.f, my = 0.f, mz = 0.f;
     for (int b = blockIdx.x; b < npblk; b += gridDim.x) {
         const BlockMap bm = pmap[b];
@@ -93,7 +98,9 @@ __global__ __launch_bounds__(TPB) void k_kick_drift(const MemberDesc* __restrict
 // since the sort (reset != 0 after a sort clears it) and sets or advances the simulated-time
 // clock read by the kernels (mode 0: clock = t, mode 1: clock += dt, mode 2: leave).
 __global__ void k_step_begin(float* __restrict__ disp, uint32_t* __restrict__ slots, int reset,
-                             double* __restrict__ clock, int mode, double t, float dt) {
+                             double* __restrict__ clock, int mode, double t, float dt,
+                             const StepCtl* __restrict__ ctl, int sort_gate) {
+    if (ctl && (ctl->idle || (sort_gate && !ctl->do_sort))) return;
     int tid = threadIdx.x;   // 64 threads
     float m[3];
 #pragma unroll
@@ -124,13 +131,21 @@ __global__ __launch_bounds__(TPB) void k_rain(const MemberDesc* __restrict__ mem
                                               const int64_t* __restrict__ dep,
                                               int64_t* __restrict__ rain_out,
                                               int64_t* __restrict__ precip,
-                                              unsigned long long* __restrict__ clipped, float dt) {
+                                              unsigned long long* __restrict__ clipped, float dt,
+                                              const StepCtl* __restrict__ ctl) {
     const BlockMap bm = pmap[blockIdx.x];
     const MemberDesc& M = mem[bm.m];
     const DevParams& P = M.P;
     const int cl = bm.base + threadIdx.x;
     if (cl >= P.ncell) return;
     const int c = M.p0 + cl;
+    if (ctl) {
+        if (ctl->idle) {   // an idle step still rotates the grid buffers
+            rain_out[c] = rain_in[c];
+            return;
+        }
+        dt = ctl->dt;
+    }
     int64_t cur = rain_in[c] + dep[c];
     if (cur < 0) {
         atomicAdd(clipped, (unsigned long long)(-cur));
@@ -166,7 +181,9 @@ __global__ __launch_bounds__(TPB) void k_rain_merge(int n, int64_t* __restrict__
 __global__ __launch_bounds__(TPB) void k_keys(const MemberDesc* __restrict__ mem,
                                               const BlockMap* __restrict__ pmap,
                                               const float4* __restrict__ pos,
-                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ ngaps) {
+                                              uint32_t* __restrict__ keys, uint32_t* __restrict__ ngaps,
+                                              const StepCtl* __restrict__ ctl) {
+    if (ctl && !ctl->do_sort) return;
     if (blockIdx.x == 0 && threadIdx.x == 0) *ngaps = 0u;   // gap list of this sort's cell table
     const BlockMap bm = pmap[blockIdx.x];
     const MemberDesc& M = mem[bm.m];
@@ -186,10 +203,12 @@ __global__ __launch_bounds__(TPB) void k_gather(int N, const uint32_t* __restric
                                                 const float4* __restrict__ vA, float4* __restrict__ vB,
                                                 const float4* __restrict__ tA, float4* __restrict__ tB,
                                                 const float* __restrict__ rA, float* __restrict__ rB,
-                                                const float4* __restrict__ aA, float4* __restrict__ aB) {
+                                                const float4* __restrict__ aA, float4* __restrict__ aB,
+                                                const StepCtl* __restrict__ ctl) {
     int i = blockIdx.x * TPB + threadIdx.x;
     if (i >= N) return;
-    uint32_t s = perm[i];
+    // a step without a sort (device controller) copies in place order: the buffers still rotate
+    uint32_t s = (ctl && !ctl->do_sort) ? (uint32_t)i : perm[i];
     pB[i] = pA[s];
     vB[i] = vA[s];
     tB[i] = tA[s];
@@ -207,7 +226,9 @@ constexpr uint32_t GAP_SERIAL = 256;
 __global__ __launch_bounds__(TPB) void k_cell_start(int N, uint32_t nkeys,
                                                     const uint32_t* __restrict__ keys,
                                                     uint32_t* __restrict__ cstart, uint4* __restrict__ gaps,
-                                                    uint32_t* __restrict__ ngaps, uint32_t gapcap) {
+                                                    uint32_t* __restrict__ ngaps, uint32_t gapcap,
+                                                    const StepCtl* __restrict__ ctl) {
+    if (ctl && !ctl->do_sort) return;
     const int i = blockIdx.x * TPB + threadIdx.x;
     if (i > N) return;
     uint32_t lo, hi;
@@ -229,7 +250,9 @@ __global__ __launch_bounds__(TPB) void k_cell_start(int N, uint32_t nkeys,
 }
 
 __global__ __launch_bounds__(TPB) void k_cell_gaps(const uint4* __restrict__ gaps, const uint32_t* __restrict__ ngaps,
-                                                   uint32_t gapcap, uint32_t* __restrict__ cstart) {
+                                                   uint32_t gapcap, uint32_t* __restrict__ cstart,
+                                                   const StepCtl* __restrict__ ctl) {
+    if (ctl && !ctl->do_sort) return;
     const uint32_t n = min(*ngaps, gapcap);
     for (uint32_t g = blockIdx.x; g < n; g += gridDim.x) {
         const uint4 r = gaps[g];
@@ -241,7 +264,8 @@ __global__ __launch_bounds__(TPB) void k_cell_gaps(const uint4* __restrict__ gap
 __global__ __launch_bounds__(TPB) void k_clamp_vel(const MemberDesc* __restrict__ mem,
                                                    const BlockMap* __restrict__ pmap,
                                                    float4* __restrict__ vel, float vmax_abs,
-                                                   float cs_frac) {
+                                                   float cs_frac, const StepCtl* __restrict__ ctl) {
+    if (ctl && !ctl->clamp) return;
     const BlockMap bm = pmap[blockIdx.x];
     const MemberDesc& M = mem[bm.m];
     const int il = bm.base + threadIdx.x;
@@ -361,26 +385,94 @@ inline int nblocks(int64_t n) { return (int)((n + TPB - 1) / TPB); }
 
 }  // namespace
 
+// ---------------------------------------------------------------------------------------------
+// Device-resident step controller: the host loop of asph_run (asph_h100.cu:958-1078) as two
+// one-wave kernels around each captured step. Same float arithmetic as the host (the expressions
+// are written in the same order), so a run under it is bit for bit asph_run's.
+// ---------------------------------------------------------------------------------------------
+__global__ void k_ctl_pre(StepCtl* __restrict__ c) {
+    if (threadIdx.x != 0) return;
+    c->clamp = 0;
+    const bool stop = c->abort || c->pause || (c->sim_end > 0.0 && c->simt >= c->sim_end) ||
+                      (c->max_steps > 0 && c->steps >= c->max_steps);
+    if (stop) {
+        c->idle = 1;
+        c->do_sort = 0;
+        return;
+    }
+    c->idle = 0;
+    c->steps += 1;
+    const bool sort = c->every || c->since_sort >= (double)c->sort_dt;   // :963
+    c->do_sort = sort ? 1 : 0;
+    if (sort) {
+        c->since_sort = 0.0;
+        c->sorts += 1;
+    }
+}
+
+__global__ void k_ctl_post(StepCtl* __restrict__ c, const uint32_t* __restrict__ maxv) {
+    if (c->idle) return;
+    const int t = threadIdx.x;   // 64 threads: max|v| of the step from the momentum kernel's slots
+    float u[3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        uint32_t v = maxv[a * 64 + t];
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) v = max(v, (uint32_t)__shfl_xor((int)v, o));
+        u[a] = __uint_as_float(v);
+    }
+    if (t != 0) return;
+    const float dt = c->dt;
+    c->simt += (double)dt;                                                             // :1038
+    c->since_sort += (double)dt;
+    c->u[0] = u[0]; c->u[1] = u[1]; c->u[2] = u[2];
+    const float dreq = c->cfl * fminf(fminf(c->dx / (c->cs + u[0]), c->dy / (c->cs + u[1])),
+                                      c->dz / (c->cs + u[2])) * 0.90f;                 // :1048-1049
+    c->dt_required = dreq;
+    float adopt = c->fixed ? dt : fminf(c->dt_cap, dreq);
+    float dtn = dt;
+    if (!c->fixed) {
+        if (adopt < c->dt_floor) {                                                     // :1054-1071
+            c->trips += 1;
+            if (c->strict || c->trips > c->max_trip) {
+                c->abort = 1;
+                c->idle = 1;
+                return;
+            }
+            c->clamp = 1;
+            adopt = c->clamp_dt;
+        } else if (adopt < 0.98f * dt) {
+            c->trips = 0;                                                              // :1072
+        }
+        dtn = adopt;
+    }
+    c->dt = dtn;
+    const float dz_move = 0.3f * c->dz / fmaxf(u[2], 0.5f);                            // :1075-1077
+    const float dh_move = 0.3f * c->dx / fmaxf(fmaxf(u[0], u[1]), 0.5f);
+    c->sort_dt = fminf(fmaxf(fminf(dz_move, dh_move), dtn), 15.0f);
+    if (c->dump_every > 0.0 && c->simt >= c->next_dump) c->pause = 1;                  // :1081
+}
+
 int particle_block_size() { return TPB; }
 
 // ---------------------------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------------------------
 void launch_step_begin(hipStream_t s, float* disp, uint32_t* disp_slots, int reset, double* clock,
-                       int clock_mode, double t, float dt) {
-    k_step_begin<<<1, 64, 0, s>>>(disp, disp_slots, reset, clock, clock_mode, t, dt);
+                       int clock_mode, double t, float dt, const StepCtl* ctl, int sort_gate) {
+    k_step_begin<<<1, 64, 0, s>>>(disp, disp_slots, reset, clock, clock_mode, t, dt, ctl, sort_gate);
 }
 
 void launch_kick_drift(const Launch& L, hipStream_t s, float4* pos, float4* vel, const float4* acc,
                        int64_t* dep, uint32_t* disp_slots, float dt) {
     const int grid = L.npblk < KD_GRID ? L.npblk : KD_GRID;
     k_kick_drift<<<grid, TPB, 0, s>>>(L.mem, L.pmap, L.npblk, pos, vel, acc, dep, disp_slots,
-                                      0.5f * dt, dt);
+                                      0.5f * dt, dt, L.ctl);
 }
 
 void launch_rain(const Launch& L, hipStream_t s, const int64_t* rain_in, const int64_t* dep,
                  int64_t* rain_out, int64_t* precip, unsigned long long* clipped, float dt) {
-    k_rain<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, rain_in, dep, rain_out, precip, clipped, dt);
+    k_rain<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, rain_in, dep, rain_out, precip, clipped, dt, L.ctl);
 }
 
 void launch_rain_merge(hipStream_t s, int n, int64_t* rain, int64_t* dep) {
@@ -388,13 +480,13 @@ void launch_rain_merge(hipStream_t s, int n, int64_t* rain, int64_t* dep) {
 }
 
 void launch_keys(const Launch& L, hipStream_t s, const float4* pos, uint32_t* keys) {
-    k_keys<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, pos, keys, L.ngaps);
+    k_keys<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, pos, keys, L.ngaps, L.ctl);
 }
 
 void launch_gather(hipStream_t s, int N, const uint32_t* perm, const float4* pA, float4* pB,
                    const float4* vA, float4* vB, const float4* tA, float4* tB, const float* rA,
-                   float* rB, const float4* aA, float4* aB) {
-    k_gather<<<nblocks(N), TPB, 0, s>>>(N, perm, pA, pB, vA, vB, tA, tB, rA, rB, aA, aB);
+                   float* rB, const float4* aA, float4* aB, const StepCtl* ctl) {
+    k_gather<<<nblocks(N), TPB, 0, s>>>(N, perm, pA, pB, vA, vB, tA, tB, rA, rB, aA, aB, ctl);
 }
 
 // Disjoint runs longer than GAP_SERIAL keys: at most (nkeys + 1) / (GAP_SERIAL + 1) + 1 of them.
@@ -402,13 +494,22 @@ uint32_t cell_gap_capacity(uint32_t nkeys) { return (nkeys + 1u) / (GAP_SERIAL +
 
 void launch_cell_start(const Launch& L, hipStream_t s, int N, uint32_t nkeys, const uint32_t* keys,
                        uint32_t* cstart) {
-    k_cell_start<<<nblocks((int64_t)N + 1), TPB, 0, s>>>(N, nkeys, keys, cstart, L.gaps, L.ngaps, L.gapcap);
-    k_cell_gaps<<<1024, TPB, 0, s>>>(L.gaps, L.ngaps, L.gapcap, cstart);
+    k_cell_start<<<nblocks((int64_t)N + 1), TPB, 0, s>>>(N, nkeys, keys, cstart, L.gaps, L.ngaps, L.gapcap,
+                                                         L.ctl);
+    k_cell_gaps<<<1024, TPB, 0, s>>>(L.gaps, L.ngaps, L.gapcap, cstart, L.ctl);
 }
 
 void launch_clamp_vel(const Launch& L, hipStream_t s, float4* vel, float vmax, float cs_frac) {
-    k_clamp_vel<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, vel, vmax, cs_frac);
+    k_clamp_vel<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, vel, vmax, cs_frac, nullptr);
 }
+
+void launch_clamp_vel_ctl(const Launch& L, hipStream_t s, float4* vel) {
+    k_clamp_vel<<<L.npblk, TPB, 0, s>>>(L.mem, L.pmap, vel, 0.f, 0.5f, L.ctl);
+}
+
+void launch_ctl_pre(hipStream_t s, StepCtl* ctl) { k_ctl_pre<<<1, 64, 0, s>>>(ctl); }
+
+void launch_ctl_post(hipStream_t s, StepCtl* ctl, const uint32_t* maxv) { k_ctl_post<<<1, 64, 0, s>>>(ctl, maxv); }
 
 void launch_audit(const Launch& L, hipStream_t s, const float4* pos, const float4* vel,
                   const float4* thq, const int64_t* rain, const int64_t* dep, const int64_t* precip,

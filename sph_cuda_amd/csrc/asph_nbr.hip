@@ -674,8 +674,18 @@ __device__ __forceinline__ void item_range(const uint32_t* __restrict__ cstart, 
 template <int S>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_density_eos_brick(
         const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
-        const uint32_t* __restrict__ nitems, int nbblk, int n_members, int csize, DensArgs A) {
+        const uint32_t* __restrict__ nitems, int nbblk, int n_members, int csize, DensArgs A0) {
     __shared__ Smem<S, false> sm;
+    DensArgs A = A0;
+    if (A.ctl) {   // device controller: its dt; an idle step only rotates the density buffers
+        if (A.ctl->idle) {
+            const int64_t n = A.ctl->n;
+            for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT)
+                A.rho_new[i] = A.rho_old[i];
+            return;
+        }
+        A.dt = A.ctl->dt;
+    }
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
     if (b < n_members && tid < 3 * 64) A.maxv[b * 192 + tid] = 0u;   // max|v| slots of member b
@@ -829,8 +839,18 @@ __device__ __forceinline__ void reduce_maxv(uint32_t* maxv, float mx, float my, 
 template <int S>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_momentum_kdk_brick(
         const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
-        const uint32_t* __restrict__ nitems, int nbblk, int csize, MomArgs A) {
+        const uint32_t* __restrict__ nitems, int nbblk, int csize, MomArgs A0) {
     __shared__ Smem<S, true> sm;
+    MomArgs A = A0;
+    if (A.ctl) {   // device controller: its dt; an idle step only rotates the velocity buffers
+        if (A.ctl->idle) {
+            const int64_t n = A.ctl->n;
+            for (int64_t i = (int64_t)blockIdx.x * BT + threadIdx.x; i < n; i += (int64_t)gridDim.x * BT)
+                A.vel_out[i] = A.vel[i];
+            return;
+        }
+        A.dt = A.ctl->dt;
+    }
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
     prologue<S, true>(A.disp, A.disp_slots, sm);
@@ -1272,7 +1292,9 @@ __global__ __launch_bounds__(256) void k_items_count(const MemberDesc* __restric
                                                      const BlockMap* __restrict__ bmap, int nbblk,
                                                      uint32_t csize, int order,
                                                      const uint32_t* __restrict__ cstart,
-                                                     uint32_t* __restrict__ meta, uint32_t* __restrict__ hist) {
+                                                     uint32_t* __restrict__ meta, uint32_t* __restrict__ hist,
+                                                     const StepCtl* __restrict__ ctl) {
+    if (ctl && !ctl->do_sort) return;
     __shared__ uint32_t lh[ITEM_BINS];
     const int t = threadIdx.x;
     if (t < ITEM_BINS) lh[t] = 0u;
@@ -1300,7 +1322,9 @@ __global__ __launch_bounds__(256) void k_items_count(const MemberDesc* __restric
 
 // One workgroup of ITEM_BINS threads: bin starts (heaviest bin first) into hist[ITEM_BINS + k],
 // the item count, and the counts cleared for the next build.
-__global__ __launch_bounds__(ITEM_BINS) void k_items_scan(uint32_t* __restrict__ hist, uint32_t* __restrict__ nitems) {
+__global__ __launch_bounds__(ITEM_BINS) void k_items_scan(uint32_t* __restrict__ hist, uint32_t* __restrict__ nitems,
+                                                          const StepCtl* __restrict__ ctl) {
+    if (ctl && !ctl->do_sort) return;
     __shared__ uint32_t v[ITEM_BINS];
     const int t = threadIdx.x;
     v[t] = hist[t];
@@ -1319,7 +1343,9 @@ __global__ __launch_bounds__(ITEM_BINS) void k_items_scan(uint32_t* __restrict__
 // Writes each brick's items at its bin's cursor: positions are reserved per workgroup and bin
 // with one global atomic, and handed out within the workgroup through LDS atomics.
 __global__ __launch_bounds__(256) void k_items_fill(int nbblk, const uint32_t* __restrict__ meta,
-                                                    uint32_t* __restrict__ hist, int2* __restrict__ items) {
+                                                    uint32_t* __restrict__ hist, int2* __restrict__ items,
+                                                    const StepCtl* __restrict__ ctl) {
+    if (ctl && !ctl->do_sort) return;
     __shared__ uint32_t lh[ITEM_BINS], base[ITEM_BINS];
     const int t = threadIdx.x;
     if (t < ITEM_BINS) lh[t] = 0u;
@@ -1350,9 +1376,9 @@ int brick_chunk_max() { return CHUNK_MAX; }
 void launch_brick_items(const Launch& L, hipStream_t s, const uint32_t* cstart) {
     const int g = (L.nbblk + 255) / 256;
     k_items_count<<<g, 256, 0, s>>>(L.mem, L.bmap, L.nbblk, (uint32_t)L.chunk, L.order, cstart, L.item_meta,
-                                    L.item_hist);
-    k_items_scan<<<1, ITEM_BINS, 0, s>>>(L.item_hist, L.nitems);
-    k_items_fill<<<g, 256, 0, s>>>(L.nbblk, L.item_meta, L.item_hist, L.items);
+                                    L.item_hist, L.ctl);
+    k_items_scan<<<1, ITEM_BINS, 0, s>>>(L.item_hist, L.nitems, L.ctl);
+    k_items_fill<<<g, 256, 0, s>>>(L.nbblk, L.item_meta, L.item_hist, L.items, L.ctl);
 }
 
 void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A) {

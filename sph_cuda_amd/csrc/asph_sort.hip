@@ -37,7 +37,8 @@ __device__ __forceinline__ uint64_t match_digit(uint32_t d, int bits, bool valid
 
 __global__ __launch_bounds__(RS_THREADS) void k_radix_hist(const uint32_t* __restrict__ kin, int N,
                                                            int shift, int bits,
-                                                           uint32_t* __restrict__ bh, int nblk) {
+                                                           uint32_t* __restrict__ bh, int nblk, const StepCtl* __restrict__ ctl) {
+    if (ctl && !ctl->do_sort) return;
     __shared__ uint32_t h[256];
     const int t = threadIdx.x;
     const int lane = t & 63;
@@ -84,7 +85,8 @@ __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t& e
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scan_digits(const uint32_t* __restrict__ bh,
                                                                   uint32_t* __restrict__ bo,
                                                                   uint32_t* __restrict__ dtot,
-                                                                  int nblk) {
+                                                                  int nblk, const StepCtl* __restrict__ ctl) {
+    if (ctl && !ctl->do_sort) return;
     __shared__ uint32_t s_w[RS_WAVES];
     const size_t row = (size_t)blockIdx.x * nblk;
     uint32_t carry = 0;
@@ -110,7 +112,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scan_digits(const uint32_t
     if (threadIdx.x == 0) dtot[blockIdx.x] = carry;
 }
 
-__global__ __launch_bounds__(RS_THREADS) void k_radix_scan_totals(uint32_t* __restrict__ dtot) {
+__global__ __launch_bounds__(RS_THREADS) void k_radix_scan_totals(uint32_t* __restrict__ dtot, const StepCtl* __restrict__ ctl) {
+    if (ctl && !ctl->do_sort) return;
     __shared__ uint32_t s_w[RS_WAVES];
     uint32_t v = dtot[threadIdx.x], excl;
     block_exclusive_scan(v, excl, s_w);
@@ -120,7 +123,8 @@ __global__ __launch_bounds__(RS_THREADS) void k_radix_scan_totals(uint32_t* __re
 __global__ __launch_bounds__(RS_THREADS) void k_radix_scatter(
     const uint32_t* __restrict__ kin, const uint32_t* __restrict__ vin, uint32_t* __restrict__ kout,
     uint32_t* __restrict__ vout, int N, int shift, int bits, const uint32_t* __restrict__ bh,
-    const uint32_t* __restrict__ bo, const uint32_t* __restrict__ dbase, int nblk) {
+    const uint32_t* __restrict__ bo, const uint32_t* __restrict__ dbase, int nblk, const StepCtl* __restrict__ ctl) {
+    if (ctl && !ctl->do_sort) return;
     __shared__ uint32_t s_key[RS_TILE];
     __shared__ uint32_t s_val[RS_TILE];
     __shared__ uint32_t s_cnt[256];
@@ -190,7 +194,7 @@ int radix_tile() { return RS_TILE; }
 
 void launch_radix_sort(hipStream_t s, int N, int end_bit, const uint32_t* keys0, uint32_t* kA,
                        uint32_t* vA, uint32_t* kB, uint32_t* vB, uint32_t* bh, uint32_t* bo,
-                       uint32_t* dtot, uint32_t** keys_sorted, uint32_t** perm) {
+                       uint32_t* dtot, uint32_t** keys_sorted, uint32_t** perm, const StepCtl* ctl) {
     const int nblk = (N + RS_TILE - 1) / RS_TILE;
     const int npass = (end_bit + 7) / 8;
     const uint32_t* kin = keys0;
@@ -200,11 +204,11 @@ void launch_radix_sort(hipStream_t s, int N, int end_bit, const uint32_t* keys0,
     for (int p = 0; p < npass; ++p) {
         int shift = 8 * p;
         int bits = min(8, end_bit - shift);
-        k_radix_hist<<<nblk, RS_THREADS, 0, s>>>(kin, N, shift, bits, bh, nblk);
-        k_radix_scan_digits<<<256, RS_THREADS, 0, s>>>(bh, bo, dtot, nblk);
-        k_radix_scan_totals<<<1, RS_THREADS, 0, s>>>(dtot);
+        k_radix_hist<<<nblk, RS_THREADS, 0, s>>>(kin, N, shift, bits, bh, nblk, ctl);
+        k_radix_scan_digits<<<256, RS_THREADS, 0, s>>>(bh, bo, dtot, nblk, ctl);
+        k_radix_scan_totals<<<1, RS_THREADS, 0, s>>>(dtot, ctl);
         k_radix_scatter<<<nblk, RS_THREADS, 0, s>>>(kin, vin, ko, vo, N, shift, bits, bh, bo, dtot,
-                                                    nblk);
+                                                    nblk, ctl);
         kin = ko;
         vin = vo;
         if (ko == kA) { ko = kB; vo = vB; } else { ko = kA; vo = vA; }

@@ -245,6 +245,23 @@ class Sim:
             L.check(rc, "asph_run")
         return rc, st
 
+    def run_device(self, sim_end: float = 0.0, max_steps: int = 0, dt: float = 0.0, fixed_dt: bool = False,
+                   dump_every: float = 0.0, dump_stride=(10, 10, 3), outdir: str | None = None,
+                   validate: bool = False, verbose: bool = False, poll: int = 16):
+        """run() with the loop controller on the device (asph_run_device): the same results,
+        the host polls every `poll` steps. Returns (rc, RunStats)."""
+        a = L.RunArgs()
+        a.sim_end, a.max_steps, a.dt, a.fixed_dt = sim_end, max_steps, dt, int(fixed_dt)
+        a.dump_every = dump_every
+        a.dump_sx, a.dump_sy, a.dump_sz = dump_stride
+        a.outdir = outdir.encode() if outdir else None
+        a.validate, a.verbose = int(validate), int(verbose)
+        st = L.RunStats()
+        rc = self.lib.asph_run_device(self.h, C.byref(a), poll, C.byref(st))
+        if rc not in (L.OK, L.CFL_ABORT):
+            L.check(rc, "asph_run_device")
+        return rc, st
+
     def clock(self):
         t, s, d = C.c_double(), C.c_double(), C.c_float()
         L.check(self.lib.asph_get_clock(self.h, C.byref(t), C.byref(s), C.byref(d)), "asph_get_clock")
