@@ -31,6 +31,7 @@ struct asph_t {
     float4 *pos = nullptr, *vel = nullptr, *thq = nullptr, *acc = nullptr;
     float4 *pos2 = nullptr, *vel2 = nullptr, *thq2 = nullptr, *acc2 = nullptr;
     float *rho = nullptr, *rho2 = nullptr, *prs = nullptr, *dthdt = nullptr;
+    float2* rinv = nullptr;   // density factors of the momentum pairs, written by the density kernel
     int64_t *rain = nullptr, *rain2 = nullptr, *dep = nullptr, *precip = nullptr;
     unsigned long long* clipped = nullptr;
     uint32_t *keys0 = nullptr, *kA = nullptr, *vA = nullptr, *kB = nullptr, *vB = nullptr;

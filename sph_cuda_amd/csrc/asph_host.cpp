@@ -113,6 +113,7 @@ int create_impl(const asph_config* cfgs, int n, void* stream, asph_t** out) {
     rc |= h->alloc(&h->acc, N); rc |= h->alloc(&h->pos2, N); rc |= h->alloc(&h->vel2, N);
     rc |= h->alloc(&h->thq2, N); rc |= h->alloc(&h->acc2, N);
     rc |= h->alloc(&h->rho, N); rc |= h->alloc(&h->rho2, N); rc |= h->alloc(&h->prs, N);
+    rc |= h->alloc(&h->rinv, N);
     rc |= h->alloc(&h->dthdt, N);
     rc |= h->alloc(&h->rain, N); rc |= h->alloc(&h->rain2, N);
     rc |= h->alloc(&h->dep, N); rc |= h->alloc(&h->precip, (size_t)h->ncol);
@@ -168,7 +169,7 @@ namespace asph_host {
 
 MomArgs mom_args(asph_t* h, float dt, int mode) {
     MomArgs A;
-    A.pos = h->pos; A.vel = h->vel; A.thq = h->thq; A.rho = h->rho; A.prs = h->prs;
+    A.pos = h->pos; A.vel = h->vel; A.thq = h->thq; A.rho = h->rho; A.rinv = h->rinv; A.prs = h->prs;
     A.cstart = h->cstart; A.acc = h->acc; A.dthdt = h->dthdt; A.vel_out = h->vel2;
     A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
     A.stats = h->stats;
@@ -223,6 +224,7 @@ void do_density_eos(asph_t* h, float dt, double simt, int bits, bool use_clock, 
         A.ctl = h->L.ctl;
         A.stats = h->stats;
         A.s12_out = h->rec_s12;
+        A.rinv = h->rinv;
         A.dt = dt; A.bits = bits;
         ProfScope ps(h, ASPH_K_DENSITY_EOS);
         launch_density_eos(h->L, h->S, h->stream, A);
@@ -993,7 +995,7 @@ DensArgs trial_dens_args(asph_t* h, float dt) {
     A.pos = h->pos; A.rho_old = h->rho; A.rho_new = h->rho2; A.thq = h->thq; A.prs = h->prs;
     A.vel = h->vel; A.cstart = h->cstart; A.keys_sorted = h->keys_sorted; A.rain = h->rain;
     A.dep = h->dep; A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
-    A.clock = nullptr; A.simt = h->simt; A.stats = nullptr; A.s12_out = nullptr;
+    A.clock = nullptr; A.simt = h->simt; A.stats = nullptr; A.s12_out = nullptr; A.rinv = h->rinv;
     A.dt = dt; A.bits = 0;
     return A;
 }

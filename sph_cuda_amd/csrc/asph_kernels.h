@@ -64,6 +64,7 @@ struct DensArgs {
     double simt;                 // then this value is used
     unsigned long long* stats;   // work counters (ASPH_C_DENS_*), or null
     float2* s12_out;             // adjoint tape: the density sums (S1, S2) per particle, or null
+    float2* rinv;                // (max(rho_new, 1e-3), its reciprocal) for the momentum kernel
     const StepCtl* ctl = nullptr;   // device controller (dt, idle), or null
     float dt;
     int bits;                    // bit 0: EOS + microphysics, bit 1: surface flux (if flagged)
@@ -74,6 +75,7 @@ struct MomArgs {
     const float4* vel;           // (vx, vy, vz, p')
     float4* thq;
     const float* rho;
+    const float2* rinv;          // (max(rho, 1e-3), its reciprocal), from the density kernel
     const float* prs;
     const uint32_t* cstart;
     float4* acc;

@@ -125,13 +125,13 @@ struct MomAcc {
 __device__ __forceinline__ void mom_pair(const DevParams& P, float rx, float ry, float rz,
                                          float dxt, float dyt, float dzt, float r2t, float4 vi,
                                          float rhoi, float inv_rhoi, float pri, float rhoj,
-                                         float4 vj, float prj, MomAcc& A, bool act = true) {
+                                         float inv_rhoj, float4 vj, float prj, MomAcc& A,
+                                         bool act = true) {
     float q = sqrt_rn(r2t) * P.inv_h;
     float kk = kgrad(P, q);
     float gx = (kk * dxt) * P.inv_dx;
     float gy = (kk * dyt) * P.inv_dy;
     float gz = (kk * dzt) * P.inv_dz;
-    float inv_rhoj = 1.0f / rhoj;
     float dp = (prj - pri) * (inv_rhoi * inv_rhoj);
     float vdx = vi.x - vj.x, vdy = vi.y - vj.y, vdz = vi.z - vj.z;
     float vdotr = __builtin_fmaf(vdz, rz, __builtin_fmaf(vdy, ry, vdx * rx));
@@ -142,13 +142,20 @@ __device__ __forceinline__ void mom_pair(const DevParams& P, float rx, float ry,
         visc = (P.visc_c * vdotr) / (((rp2 + P.eta2) * (rbar * rbar)) * rhoj);
     }
     float br = dp + visc;
-    // act = false: a placeholder pair of an idle queue lane, evaluated and dropped
-    A.ax = act ? __builtin_fmaf(-br, gx, A.ax) : A.ax;
-    A.ay = act ? __builtin_fmaf(-br, gy, A.ay) : A.ay;
-    A.az = act ? __builtin_fmaf(-br, gz, A.az) : A.az;
-    A.avx = act ? __builtin_fmaf(-visc, gx, A.avx) : A.avx;
-    A.avy = act ? __builtin_fmaf(-visc, gy, A.avy) : A.avy;
-    A.avz = act ? __builtin_fmaf(-visc, gz, A.avz) : A.avz;
+    // act = false: a placeholder pair of an idle queue lane, evaluated and dropped. Its factors
+    // are replaced by zeros, so the sums take fma(-0, 0, a) = a (a sum that starts at +0 never
+    // holds -0) and nothing of the placeholder, even a non-finite value, reaches them.
+    br = act ? br : 0.f;
+    visc = act ? visc : 0.f;
+    gx = act ? gx : 0.f;
+    gy = act ? gy : 0.f;
+    gz = act ? gz : 0.f;
+    A.ax = __builtin_fmaf(-br, gx, A.ax);
+    A.ay = __builtin_fmaf(-br, gy, A.ay);
+    A.az = __builtin_fmaf(-br, gz, A.az);
+    A.avx = __builtin_fmaf(-visc, gx, A.avx);
+    A.avy = __builtin_fmaf(-visc, gy, A.avy);
+    A.avz = __builtin_fmaf(-visc, gz, A.avz);
 }
 
 template <int S>
@@ -175,8 +182,9 @@ __device__ __forceinline__ void mom_sum_global(const DevParams& P, uint32_t k0, 
                     float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
                     // j == i has r2t = 0 and is excluded here as by `if (j==i) continue` (:452)
                     if (r2t >= P.lim || r2t < 1e-12f) continue;
-                    mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri,
-                             fmaxf(rho[j], 1e-3f), vel[j], prs[j], A);
+                    const float rhoj = fmaxf(rho[j], 1e-3f);
+                    mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, rhoj,
+                             1.0f / rhoj, vel[j], prs[j], A);
                 }
             }
         }
@@ -188,7 +196,9 @@ __device__ __forceinline__ void mom_sum_global(const DevParams& P, uint32_t k0, 
 // ---------------------------------------------------------------------------------------------
 // density ratio + clamp (:331-332), surface flux (:606-626), EOS + microphysics (:353-401),
 // Kessler accretion and rain evaporation. Writes rho_new, thq, prs and p' into vel.w (the
-// momentum kernel reads (v, p') of a neighbour with one load; the reference leaves vel.w unused).
+// momentum kernel reads (v, p') of a neighbour with one load; the reference leaves vel.w unused),
+// and (max(rho, 1e-3), 1 / max(rho, 1e-3)) into rinv, the two density factors of a momentum pair
+// (asph_h100.cu:455-458), computed once per particle instead of once per pair.
 struct DensConst {
     float auto_f, Ts, qs_sfc;
     int p0;          // member's first particle (= first cell)
@@ -203,6 +213,10 @@ __device__ __forceinline__ void dens_epilogue(const DevParams& P, const DensArgs
     float rr = S2 > 1e-20f ? S1 / S2 : rref;
     float rnew = fminf(fmaxf(rr, 0.5f * rref), 2.0f * rref);
     A.rho_new[i] = rnew;
+    {
+        const float rc = fmaxf(rnew, 1e-3f);
+        A.rinv[i] = make_float2(rc, 1.0f / rc);
+    }
     if (A.s12_out) A.s12_out[i] = make_float2(S1, S2);
     if (!(A.bits & 1)) return;
 
@@ -778,19 +792,25 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
             if (WRAP) min_image(P, rx, ry);
         };
         auto flush = [&]() {
-            // lanes past their queue evaluate a placeholder (slot 0, as in the density kernel)
-            for (int k = 0; wave_any(k < qn); ++k) {
+            // lanes past their queue evaluate a placeholder (slot 0, as in the density kernel).
+            // Called only when some lane has queued a pair; the loop tests at the bottom, so the
+            // sums stay in place across iterations.
+            MomAcc L = M;
+            int k = 0;
+            do {
                 const bool act = k < qn;
                 float4 pj = sm.p[act ? sm.q[k][tid] : 0];
                 const uint32_t jg = __float_as_uint(pj.w);
                 float4 vj = A.vel[jg];                 // (vx, vy, vz, p'_j)
-                float rhoj = fmaxf(A.rho[jg], 1e-3f);
+                const float2 rj = A.rinv[jg];          // (max(rho_j, 1e-3), its reciprocal)
                 float rx, ry, rz;
                 diff(pj, rx, ry, rz);
                 float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
                 float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
-                mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, rhoj, vj, vj.w, M, act);
-            }
+                mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, rj.x, rj.y, vj, vj.w, L, act);
+                ++k;
+            } while (wave_any(k < qn));
+            M = L;
             npair += qn;
             qn = 0;
         };
@@ -810,7 +830,7 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
                     if (wave_any(qn == QK)) flush();
                 });
             }
-            flush();
+            if (wave_any(qn > 0)) flush();
         }
         if (active) mom_epilogue(P, A, (int)i, pi, vi, ti, pri, M, mx, my, mz);
         if (A.stats) {
