@@ -954,14 +954,17 @@ __device__ __forceinline__ int fold(int d, int n) {   // (-n, n) -> (-n/2, n/2]
 }
 __device__ __forceinline__ int wrap1(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
 
-// Whether cell (cx, cy, cz) lies in the stencil of current cell (ix, iy, iz), asph_h100.cu:297-312.
+// Whether cell (cx, cy, cz), 0 <= cz < nz, lies in the stencil of current cell (ix, iy, iz),
+// asph_h100.cu:297-312. The z range clipped at the walls, [max(iz - S, 0), min(iz + S, nz - 1)],
+// holds such a cz exactly when |cz - iz| <= S; across a periodic axis |fold(d)| <= S holds exactly
+// when |d| <= S or |d| >= n - S (|d| < n).
+__device__ __forceinline__ bool near1(int d, int n, int S) {
+    const int a = abs(d);
+    return a <= S || a >= n - S;
+}
 __device__ __forceinline__ bool in_stencil(const DevParams& P, int ix, int iy, int iz, int cx, int cy, int cz) {
     const int S = P.S;
-    const bool interior = (iz >= S && iz < P.nz - S);
-    const int z0 = interior ? iz - S : max(iz - S, 0);
-    const int z1 = interior ? iz + S : min(iz + S, P.nz - 1);
-    if (cz < z0 || cz > z1) return false;
-    return abs(fold(cx - ix, P.nx)) <= S && abs(fold(cy - iy, P.ny)) <= S;
+    return abs(cz - iz) <= S && near1(cx - ix, P.nx, S) && near1(cy - iy, P.ny, S);
 }
 
 // Distance [cells] from a point at offset t of a cell's centre to the cell's box widened by d,
@@ -1042,7 +1045,11 @@ __device__ __forceinline__ void walk_global(const DevParams& P, uint32_t k0, con
 }
 
 // Halo walk of slab s: f(slot, own) for the staged particles of the halo cells within the margin
-// of i's listed cell (which lies in the brick) whose box meets the support.
+// of i's listed cell (which lies in the brick) whose box [c - d, c + 1 + d] meets the support, in
+// ascending (z, y, x). Per row the cells are culled to the chord of the support (as the forward
+// walk does), in the brick's unwrapped frame: i's position and current cell are shifted by a period
+// when i has crossed a periodic face since the sort. The window (2 S + 3 cells per axis, at most
+// nx, ny) holds each cell once, so the unwrapped distances are the nearest-image ones.
 template <int S, class F>
 __device__ __forceinline__ void walk_halo(const DevParams& P, const BrickGeo& g, const Smem<S, true>& sm, int s,
                                           const Part& a, F f) {
@@ -1052,23 +1059,32 @@ __device__ __forceinline__ void walk_halo(const DevParams& P, const BrickGeo& g,
     const int L2 = g.Hx * g.Hy;
     const int base = sm.off[sm.slab[s] * L2];
     const int za = max(hz - mz, (int)sm.slab[s]), zb = min(hz + mz, (int)sm.slab[s + 1] - 1);
+    const int sx = a.ix - a.lx > P.nx / 2 ? -P.nx : (a.lx - a.ix > P.nx / 2 ? P.nx : 0);
+    const int sy = a.iy - a.ly > P.ny / 2 ? -P.ny : (a.ly - a.iy > P.ny / 2 ? P.ny : 0);
+    const float xu = a.xc + (float)sx, yu = a.yc + (float)sy;
+    const int oxl = a.ix + sx - S - g.hx0, oxh = a.ix + sx + S - g.hx0;   // own x range (halo frame)
+    const int iyu = a.iy + sy;
     for (int z = za; z <= zb; ++z) {
         const int cz = g.hz0 + z;
         if (cz < 0 || cz >= P.nz) continue;
         const float ez = box_gap(a.zc - ((float)cz + 0.5f), sm.delta[2], 0);
         const float ez2 = ez * ez;
         if (ez2 >= limc) continue;
+        const bool oz = abs(cz - a.iz) <= S;
         for (int y = hy - my; y <= hy + my; ++y) {
-            const int cy = wrap1(g.hy0 + y, P.ny);
-            const float ey = box_gap(a.yc - ((float)cy + 0.5f), sm.delta[1], P.ny);
+            const int cyu = g.hy0 + y;
+            const float ey = box_gap(yu - ((float)cyu + 0.5f), sm.delta[1], 0);
             const float eyz = __builtin_fmaf(ey, ey, ez2);
             if (eyz >= limc) continue;
+            // a culling bound only: the 1e-3-cell margin covers the 1-ulp hardware square root
+            const float ext = __builtin_amdgcn_sqrtf(limc - eyz) + DELTA_EPS;
+            const int xl = max(hx - mx, (int)ceilf(xu - ext - 1.0f - sm.delta[0]) - g.hx0);
+            const int xh = min(hx + mx, (int)floorf(xu + ext + sm.delta[0]) - g.hx0);
+            if (xl > xh) continue;
+            const bool oyz = oz && abs(cyu - iyu) <= S;
             const int row = (z * g.Hy + y) * g.Hx;
-            for (int x = hx - mx; x <= hx + mx; ++x) {
-                const int cx = wrap1(g.hx0 + x, P.nx);
-                const float ex = box_gap(a.xc - ((float)cx + 0.5f), sm.delta[0], P.nx);
-                if (__builtin_fmaf(ex, ex, eyz) >= limc) continue;
-                const bool own = in_stencil(P, a.ix, a.iy, a.iz, cx, cy, cz);
+            for (int x = xl; x <= xh; ++x) {
+                const bool own = oyz && x >= oxl && x <= oxh;
                 const int p0 = (int)sm.off[row + x] - base, p1 = (int)sm.off[row + x + 1] - base;
                 for (int p = p0; p < p1; ++p) f(p, own);
             }
