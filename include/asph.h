@@ -375,8 +375,8 @@ typedef struct asph_tune_report {
 int asph_set_tuning(asph_t* h, const asph_tuning* t);
 int asph_get_tuning(asph_t* h, asph_tuning* t);
 /* Times the two neighbour kernels for every candidate shape on the current sorted,
- * force-evaluated state (reps launches each, after one warm-up), keeps the fastest (the
- * heaviest-first order unless brick order is more than 3% faster) and reports all. The state
+ * force-evaluated state (reps launches each, after one warm-up), keeps the fastest (or the
+ * largest chunk in heaviest-first order within 3% of it) and reports all. The state
  * (including acc, dthdt and max|v|) is unchanged. Allocates scratch for the duration. */
 int asph_autotune(asph_t* h, int32_t reps, asph_tune_report* rep);
 

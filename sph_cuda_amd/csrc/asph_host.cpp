@@ -1072,15 +1072,19 @@ int asph_autotune(asph_t* h, int32_t reps, asph_tune_report* rep) {
         if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
         ms[k] = (double)t / reps;
     }
-    // fastest shape; heaviest-first order is kept unless brick order beats it by more than 3%:
-    // on a uniform state the two tie, and only the heaviest-first order protects the launch
-    // from the tail of dense regions that develop later
-    size_t best = 0, best1 = 0;
-    for (size_t k = 1; k < cand.size(); ++k) {
+    // Fastest shape, except that the largest chunk in heaviest-first order is kept when it is
+    // within 3% of the fastest: on a near-uniform state the shapes tie, and only the heaviest-first
+    // order and large chunks protect the launch from the dense regions that develop later (the
+    // long workgroups start first, and fewer items restage a dense brick's halo).
+    size_t best = 0;
+    for (size_t k = 1; k < cand.size(); ++k)
         if (ms[k] < ms[best]) best = k;
-        if (cand[k].order == 1 && (cand[best1].order != 1 || ms[k] < ms[best1])) best1 = k;
-    }
-    if (cand[best].order == 0 && cand[best1].order == 1 && ms[best1] <= 1.03 * ms[best]) best = best1;
+    size_t keep1 = cand.size();
+    for (size_t k = 0; k < cand.size(); ++k)
+        if (cand[k].order == 1 && ms[k] <= 1.03 * ms[best] &&
+            (keep1 == cand.size() || cand[k].chunk > cand[keep1].chunk))
+            keep1 = k;
+    if (keep1 < cand.size()) best = keep1;
     if (e != hipSuccess) {
         rc = fail("autotune: %s", hipGetErrorString(e));
         h->L.chunk = keep.chunk;
