@@ -45,7 +45,7 @@ constexpr int NB = 9;              // Morton bits per brick: 8 x 8 x 8 cells
 constexpr int BS = 8;              // brick edge in cells
 constexpr int BT = 512;            // threads per workgroup (one per brick cell)
 constexpr int BW = BT / 64;        // wavefronts per workgroup
-constexpr int QK = 8;              // queued in-support pairs per thread
+constexpr int QK = 12;             // queued in-support pairs per thread
 constexpr int CHUNK_MIN = 256;     // bounds of the work-item size (tuned at run time)
 constexpr int CHUNK_MAX = 1024;
 constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cells]
