@@ -169,8 +169,8 @@ def valu_probe():
 def device_loop_bench(steps=120, poll=32, reps=3):
     """The reference loop at the 32^3 dry workload (preset 1, CFL-controlled dt): the host loop
     (asph_run, one host synchronisation per step for max|v|) against the same loop with its
-    controller on the device (asph_run_device, a captured graph and a poll every `poll` steps).
-    Best of `reps`; the final states must agree bit for bit."""
+    controller on the device (asph_run_device, a captured graph and a poll every `poll` steps),
+    each after the autotuner. Best of `reps`; the final states must agree bit for bit."""
     import numpy as np
 
     import sph_cuda_amd as s
@@ -185,6 +185,7 @@ def device_loop_bench(steps=120, poll=32, reps=3):
                 g.set_state(st)
                 g.sort()
                 g.force(dt)
+                g.autotune(reps=2)
                 g.synchronize()
                 t0 = time.perf_counter()
                 if name == "host":
