@@ -370,13 +370,16 @@ typedef struct asph_tune_report {
     asph_tuning cand[ASPH_TUNE_MAX];
     double ms[ASPH_TUNE_MAX];        /* mean ms of density + momentum on the current state */
     int32_t best;                    /* index of the chosen candidate */
+    uint32_t items[ASPH_TUNE_MAX];   /* work items (workgroups) of each candidate on this state */
+    int32_t cus;                     /* compute units of the device */
 } asph_tune_report;
 
 int asph_set_tuning(asph_t* h, const asph_tuning* t);
 int asph_get_tuning(asph_t* h, asph_tuning* t);
 /* Times the two neighbour kernels for every candidate shape on the current sorted,
- * force-evaluated state (reps launches each, after one warm-up), keeps the fastest (or the
- * largest chunk in heaviest-first order within 3% of it) and reports all. The state
+ * force-evaluated state (reps launches each, after one warm-up) and keeps the fastest; among
+ * the heaviest-first candidates within 3% of it, it keeps instead the largest chunk that gives at
+ * least two work items per CU, or the smallest chunk when none does. Reports all. The state
  * (including acc, dthdt and max|v|) is unchanged. Allocates scratch for the duration. */
 int asph_autotune(asph_t* h, int32_t reps, asph_tune_report* rep);
 

@@ -122,7 +122,7 @@ TUNE_MAX = 16
 
 class TuneReport(C.Structure):
     _fields_ = [("n", C.c_int32), ("cand", Tuning * TUNE_MAX), ("ms", C.c_double * TUNE_MAX),
-                ("best", C.c_int32)]
+                ("best", C.c_int32), ("items", C.c_uint32 * TUNE_MAX), ("cus", C.c_int32)]
 
 
 P = C.c_void_p

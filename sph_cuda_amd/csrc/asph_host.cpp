@@ -1053,6 +1053,11 @@ int asph_autotune(asph_t* h, int32_t reps, asph_tune_report* rep) {
     drop_graphs(h);
     int rc = ASPH_OK;
     std::vector<double> ms(cand.size(), 0.0);
+    std::vector<uint32_t> nit(cand.size(), 0u);
+    int dev = 0, ncu = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        ncu = 256;
     const float dt = 0.1f;   // enters only the skipped epilogue
     hipError_t e = hipMemcpyAsync(acc_save, h->acc, N * sizeof(float4), hipMemcpyDeviceToDevice, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(dth_save, h->dthdt, N * sizeof(float), hipMemcpyDeviceToDevice, h->stream);
@@ -1070,20 +1075,25 @@ int asph_autotune(asph_t* h, int32_t reps, asph_tune_report* rep) {
         e = hipEventSynchronize(e1);
         float t = 0.f;
         if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        if (e == hipSuccess) e = hipMemcpy(&nit[k], h->L.nitems, sizeof(uint32_t), hipMemcpyDeviceToHost);
         ms[k] = (double)t / reps;
     }
-    // Fastest shape, except that the largest chunk in heaviest-first order is kept when it is
-    // within 3% of the fastest: on a near-uniform state the shapes tie, and only the heaviest-first
-    // order and large chunks protect the launch from the dense regions that develop later (the
-    // long workgroups start first, and fewer items restage a dense brick's halo).
+    // Fastest shape, except that among the heaviest-first shapes within 3% of it the largest chunk
+    // that still gives two workgroups per CU is kept (or, when none does, the smallest chunk). On a
+    // near-uniform state the shapes tie: in a large domain, large chunks and the heaviest-first
+    // order then protect the launch from the dense regions that develop later (the long
+    // workgroups start first, and fewer items restage a dense brick's halo); in a small one,
+    // a launch is as long as one workgroup's chain, and small chunks shorten it.
     size_t best = 0;
     for (size_t k = 1; k < cand.size(); ++k)
         if (ms[k] < ms[best]) best = k;
-    size_t keep1 = cand.size();
-    for (size_t k = 0; k < cand.size(); ++k)
-        if (cand[k].order == 1 && ms[k] <= 1.03 * ms[best] &&
-            (keep1 == cand.size() || cand[k].chunk > cand[keep1].chunk))
-            keep1 = k;
+    size_t keep1 = cand.size(), small1 = cand.size();
+    for (size_t k = 0; k < cand.size(); ++k) {
+        if (cand[k].order != 1 || ms[k] > 1.03 * ms[best]) continue;
+        if (nit[k] >= 2u * (uint32_t)ncu && (keep1 == cand.size() || cand[k].chunk > cand[keep1].chunk)) keep1 = k;
+        if (small1 == cand.size() || cand[k].chunk < cand[small1].chunk) small1 = k;
+    }
+    if (keep1 == cand.size()) keep1 = small1;
     if (keep1 < cand.size()) best = keep1;
     if (e != hipSuccess) {
         rc = fail("autotune: %s", hipGetErrorString(e));
@@ -1109,8 +1119,10 @@ int asph_autotune(asph_t* h, int32_t reps, asph_tune_report* rep) {
         for (int32_t k = 0; k < rep->n; ++k) {
             rep->cand[k] = cand[k];
             rep->ms[k] = ms[k];
+            rep->items[k] = nit[k];
         }
         rep->best = (int32_t)best;
+        rep->cus = ncu;
     }
     if (rc == ASPH_OK) LAUNCHCHK();
     return rc;

@@ -45,8 +45,10 @@ constexpr int NB = 9;              // Morton bits per brick: 8 x 8 x 8 cells
 constexpr int BS = 8;              // brick edge in cells
 constexpr int BT = 512;            // threads per workgroup (one per brick cell)
 constexpr int BW = BT / 64;        // wavefronts per workgroup
-constexpr int QK = 12;             // queued in-support pairs per thread (even: drained in pairs)
-static_assert(QK % 2 == 0, "the queue drains take two entries per iteration");
+constexpr int QK = 12;             // queued in-support pairs per thread
+constexpr int DU = 2;              // queue entries per drain iteration, density
+constexpr int MU = 3;              // and momentum
+static_assert(QK % DU == 0 && QK % MU == 0, "a drain iteration never reads past the queue");
 constexpr int CHUNK_MIN = 256;     // bounds of the work-item size (tuned at run time)
 constexpr int CHUNK_MAX = 1024;
 constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cells]
@@ -627,22 +629,26 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
         auto flush = [&]() {
             // lanes past their queue evaluate a placeholder (slot 0 of the staged slab, which
             // holds a particle whenever any lane has queued one) and drop it: no divergent region
-            // Two queue entries per iteration: both staged positions are read before either
-            // pair is evaluated (the sums still take them in queue order).
-            for (int k = 0; wave_any(k < qn); k += 2) {
-                const bool act0 = k < qn, act1 = k + 1 < qn;   // k + 1 < QK: k is even
-                const float4 pj0 = sm.p[act0 ? sm.q[k][tid] : 0];
-                const float4 pj1 = sm.p[act1 ? sm.q[k + 1][tid] : 0];
-                auto pair = [&](float4 pj, bool act) {
+            // DU queue entries per iteration: all their staged positions are read before the
+            // pairs are evaluated (the sums still take them in queue order).
+            for (int k = 0; wave_any(k < qn); k += DU) {
+                bool act[DU];
+                float4 pj[DU];
+#pragma unroll
+                for (int u = 0; u < DU; ++u) {   // k + u < QK: k is a multiple of DU
+                    act[u] = k + u < qn;
+                    pj[u] = sm.p[act[u] ? sm.q[k + u][tid] : 0];
+                }
+#pragma unroll
+                for (int u = 0; u < DU; ++u) {
+                    if (u > 0 && !wave_any(act[u])) break;
                     float dxm, dym, dzm;
-                    diff(pj, dxm, dym, dzm);
+                    diff(pj[u], dxm, dym, dzm);
                     float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
                     float W = fker(sqrt_rn(r2) * P.inv_h);
-                    S1 = act ? S1 + W : S1;
-                    S2 = act ? S2 + W * pj.w : S2;
-                };
-                pair(pj0, act0);
-                if (wave_any(act1)) pair(pj1, act1);
+                    S1 = act[u] ? S1 + W : S1;
+                    S2 = act[u] ? S2 + W * pj[u].w : S2;
+                }
             }
             npair += qn;
             qn = 0;
@@ -805,25 +811,30 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
             // sums stay in place across iterations.
             MomAcc L = M;
             int k = 0;
-            // Two queue entries per iteration: the operands of both pairs are loaded before either
-            // is evaluated (the sums still take them in queue order).
+            // MU queue entries per iteration: the operands of all its pairs are loaded before they
+            // are evaluated (the sums still take them in queue order).
             do {
-                const bool act0 = k < qn, act1 = k + 1 < qn;   // k + 1 < QK: k is even
-                const float4 pj0 = sm.p[act0 ? sm.q[k][tid] : 0];
-                const float4 pj1 = sm.p[act1 ? sm.q[k + 1][tid] : 0];
-                const float4 vj0 = A.vel[__float_as_uint(pj0.w)], vj1 = A.vel[__float_as_uint(pj1.w)];
-                const float2 rj0 = A.rinv[__float_as_uint(pj0.w)], rj1 = A.rinv[__float_as_uint(pj1.w)];
-                // vj = (vx, vy, vz, p'_j), rj = (max(rho_j, 1e-3), its reciprocal)
-                auto pair = [&](float4 pj, float4 vj, float2 rj, bool act) {
+                bool act[MU];
+                float4 pj[MU], vj[MU];   // vj = (vx, vy, vz, p'_j)
+                float2 rj[MU];           // (max(rho_j, 1e-3), its reciprocal)
+#pragma unroll
+                for (int u = 0; u < MU; ++u) {   // k + u < QK: k is a multiple of MU
+                    act[u] = k + u < qn;
+                    pj[u] = sm.p[act[u] ? sm.q[k + u][tid] : 0];
+                    vj[u] = A.vel[__float_as_uint(pj[u].w)];
+                    rj[u] = A.rinv[__float_as_uint(pj[u].w)];
+                }
+#pragma unroll
+                for (int u = 0; u < MU; ++u) {
+                    if (u > 0 && !wave_any(act[u])) break;
                     float rx, ry, rz;
-                    diff(pj, rx, ry, rz);
+                    diff(pj[u], rx, ry, rz);
                     float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
                     float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
-                    mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, rj.x, rj.y, vj, vj.w, L, act);
-                };
-                pair(pj0, vj0, rj0, act0);
-                if (wave_any(act1)) pair(pj1, vj1, rj1, act1);
-                k += 2;
+                    mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, rj[u].x, rj[u].y, vj[u],
+                             vj[u].w, L, act[u]);
+                }
+                k += MU;
             } while (wave_any(k < qn));
             M = L;
             npair += qn;

@@ -324,9 +324,9 @@ class Sim:
         Returns {"best": {...}, "trials": [(chunk, order, ms), ...]}."""
         r = L.TuneReport()
         L.check(self.lib.asph_autotune(self.h, reps, C.byref(r)), "asph_autotune")
-        trials = [(r.cand[k].chunk, r.cand[k].order, r.ms[k]) for k in range(r.n)]
+        trials = [(r.cand[k].chunk, r.cand[k].order, r.ms[k], r.items[k]) for k in range(r.n)]
         return {"best": {"chunk": r.cand[r.best].chunk, "order": r.cand[r.best].order},
-                "trials": trials}
+                "trials": trials, "cus": r.cus}
 
     def set_counters(self, on: bool):
         L.check(self.lib.asph_set_counters(self.h, 1 if on else 0), "asph_set_counters")

@@ -57,11 +57,19 @@ def test_autotune_keeps_state_and_steps():
     print(f"[case] autotune {rep}", flush=True)
     assert len(rep["trials"]) == 10 and all(t[2] > 0 for t in rep["trials"])
     assert b.tuning() == rep["best"]
-    # the rule: the largest chunk in heaviest-first order within 3% of the fastest, else the fastest
+    # the rule: among the heaviest-first shapes within 3% of the fastest, the largest chunk with
+    # two work items per CU, or the smallest when none has; the fastest when none is that close
     fastest = min(t[2] for t in rep["trials"])
     near = [t for t in rep["trials"] if t[1] == 1 and t[2] <= 1.03 * fastest]
-    want = max(near, key=lambda t: t[0]) if near else min(rep["trials"], key=lambda t: t[2])
+    full = [t for t in near if t[3] >= 2 * rep["cus"]]
+    if full:
+        want = max(full, key=lambda t: t[0])
+    elif near:
+        want = min(near, key=lambda t: t[0])
+    else:
+        want = min(rep["trials"], key=lambda t: t[2])
     assert (rep["best"]["chunk"], rep["best"]["order"]) == (want[0], want[1])
+    assert all(t[3] > 0 for t in rep["trials"]) and rep["cus"] > 0
     after = b.get_state()
     acc1, prs1, dth1 = b.get_aux()
     for f in FIELDS:
