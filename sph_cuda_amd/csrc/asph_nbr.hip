@@ -579,7 +579,8 @@ __device__ __forceinline__ int walk_rows(const DevParams& P, const BrickGeo& g, 
             const int row = (rowz + (jy - g.hy0)) * g.Hx - g.hx0;
             const int a = off[row + xlo] - base, e = off[row + xhi + 1] - base;
             ncand += e - a;
-            for (int p = a; p < e; ++p) cand(p);
+            // two candidates per step (the second absent at the end of an odd span)
+            for (int p = a; p < e; p += 2) cand(p, p + 1 < e);
         }
     }
     return ncand;
@@ -656,16 +657,24 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
         for (int s = 0; s < ns; ++s) {
             if (ns > 1) stage_slab<S, false>(P, k0, g, s, A.pos, A.cstart, sm, rec);
             if (active) {
-                ncand += walk_rows<S>(P, g, sm.delta, sm.off, sm.slab, s, cc, [&](int p) {
-                    float4 pj = sm.p[p];
+                // candidates p and p + 1 (if two): both positions are read before either test;
+                // the queue takes them in order and is drained before it could overflow
+                ncand += walk_rows<S>(P, g, sm.delta, sm.off, sm.slab, s, cc, [&](int p, bool two) {
+                    const float4 pj0 = sm.p[p], pj1 = sm.p[two ? p + 1 : p];
                     float dxm, dym, dzm;
-                    diff(pj, dxm, dym, dzm);
-                    float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
-                    if (r2 < P.lim) {
+                    diff(pj0, dxm, dym, dzm);
+                    const float r20 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+                    diff(pj1, dxm, dym, dzm);
+                    const float r21 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+                    if (r20 < P.lim) {
                         sm.q[qn][tid] = (uint16_t)p;
                         ++qn;
                     }
-                    if (wave_any(qn == QK)) flush();
+                    if (two && r21 < P.lim) {
+                        sm.q[qn][tid] = (uint16_t)(p + 1);
+                        ++qn;
+                    }
+                    if (wave_any(qn > QK - 2)) flush();
                 });
             }
             flush();
@@ -843,17 +852,25 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
         for (int s = 0; s < ns; ++s) {
             if (ns > 1) stage_slab<S, true>(P, k0, g, s, A.pos, A.cstart, sm, rec);
             if (active) {
-                ncand += walk_rows<S>(P, g, sm.delta, sm.off, sm.slab, s, cc, [&](int p) {
-                    float4 pj = sm.p[p];
+                // candidates p and p + 1 (if two), as in the density kernel
+                ncand += walk_rows<S>(P, g, sm.delta, sm.off, sm.slab, s, cc, [&](int p, bool two) {
+                    const float4 pj0 = sm.p[p], pj1 = sm.p[two ? p + 1 : p];
                     float rx, ry, rz;
-                    diff(pj, rx, ry, rz);
+                    diff(pj0, rx, ry, rz);
                     float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
-                    float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
-                    if (r2t < P.lim && r2t >= 1e-12f) {
+                    const float r20 = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
+                    diff(pj1, rx, ry, rz);
+                    dxt = rx * P.inv_dx; dyt = ry * P.inv_dy; dzt = rz * P.inv_dz;
+                    const float r21 = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
+                    if (r20 < P.lim && r20 >= 1e-12f) {
                         sm.q[qn][tid] = (uint16_t)p;
                         ++qn;
                     }
-                    if (wave_any(qn == QK)) flush();
+                    if (two && r21 < P.lim && r21 >= 1e-12f) {
+                        sm.q[qn][tid] = (uint16_t)(p + 1);
+                        ++qn;
+                    }
+                    if (wave_any(qn > QK - 2)) flush();
                 });
             }
             if (wave_any(qn > 0)) flush();
