@@ -48,7 +48,7 @@ constexpr int BW = BT / 64;        // wavefronts per workgroup
 constexpr int QK = 12;             // queued in-support pairs per thread
 constexpr int DU = 2;              // queue entries per drain iteration, density
 constexpr int MU = 3;              // and momentum
-static_assert(QK % DU == 0 && QK % MU == 0, "a drain iteration never reads past the queue");
+static_assert(QK % DU == 0 && QK % MU == 0 && QK % 2 == 0, "a drain iteration never reads past the queue");
 constexpr int CHUNK_MIN = 256;     // bounds of the work-item size (tuned at run time)
 constexpr int CHUNK_MAX = 1024;
 constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cells]
@@ -1173,17 +1173,29 @@ __device__ __forceinline__ MomI mom_load(const AdjTape& T, const AdjWork& W, uin
     return m;
 }
 
-__device__ __forceinline__ void mom_partner(const DevParams& P, const AdjTape& T, const AdjWork& W, MomI& m,
-                                            uint32_t j, float4 xj, bool own, bool rev) {
+// The partner's operands: (v, p') and density of the step, and (rev) its seeds.
+struct MomJ {
+    float4 v, A, AV;
+    float rho;
+};
+__device__ __forceinline__ MomJ mom_operands(const AdjTape& T, const AdjWork& W, uint32_t j, bool rev) {
+    MomJ o;
+    o.v = T.v1[j];
+    o.rho = T.r1[j];
+    // a pair outside one role enters with zero seeds for that orientation
+    o.A = make_float4(0.f, 0.f, 0.f, 0.f);
+    o.AV = o.A;
+    if (rev) { o.A = W.Ab[j]; o.AV = W.AVb[j]; }
+    return o;
+}
+
+__device__ __forceinline__ void mom_partner(const DevParams& P, MomI& m, float4 xj, const MomJ& o, bool own) {
     float rx = m.xi.x - xj.x, ry = m.xi.y - xj.y, rz = m.xi.z - xj.z;
     min_image(P, rx, ry);
-    const float4 vj4 = T.v1[j];
+    const float4 vj4 = o.v;
     const float vj[3] = {vj4.x, vj4.y, vj4.z};
-    const float rho_j = fmaxf(T.r1[j], 1e-3f);
-    // a pair outside one role enters with zero seeds for that orientation
-    float4 Aj4 = make_float4(0.f, 0.f, 0.f, 0.f), AVj4 = Aj4;
-    if (rev) { Aj4 = W.Ab[j]; AVj4 = W.AVb[j]; }
-    const float Aj[3] = {Aj4.x, Aj4.y, Aj4.z}, AVj[3] = {AVj4.x, AVj4.y, AVj4.z};
+    const float rho_j = fmaxf(o.rho, 1e-3f);
+    const float Aj[3] = {o.A.x, o.A.y, o.A.z}, AVj[3] = {o.AV.x, o.AV.y, o.AV.z};
     const float z3[3] = {0.f, 0.f, 0.f};
     adj::mom_pair_vjp2<float>(P, rx, ry, rz, m.vi, vj, m.rho_i, rho_j, m.p_i, vj4.w, own ? m.Ai : z3,
                               own ? m.AVi : z3, Aj, AVj, m.xb, m.vb, m.rb, m.pb);
@@ -1206,7 +1218,7 @@ __device__ __forceinline__ void mom_global(const DevParams& P, uint32_t k0, cons
         float rx, ry, rz;
         if (!support(P, m.xi, xj, rx, ry, rz)) return;
         const bool rev = rev_role(P, xj, a);
-        if (own || rev) mom_partner(P, T, W, m, j, xj, own, rev);
+        if (own || rev) mom_partner(P, m, xj, mom_operands(T, W, j, rev), own);
     });
     mom_store(W, i, m);
 }
@@ -1231,16 +1243,28 @@ __device__ __forceinline__ DensI dens_load(const AdjTape& T, const AdjWork& W, u
 }
 
 // W and its gradient are even and odd in r, so one evaluation serves both orientations.
-__device__ __forceinline__ void dens_partner(const DevParams& P, const AdjTape& T, const AdjWork& W, DensI& d,
-                                             uint32_t j, float4 xj, bool own, bool rev) {
+// The partner's operands: its start-of-step density (own) and density-sum seeds (rev).
+struct DensJ {
+    float r0;
+    float2 sb;
+};
+__device__ __forceinline__ DensJ dens_operands(const AdjTape& T, const AdjWork& W, uint32_t j, bool own, bool rev) {
+    DensJ o;
+    o.r0 = own ? T.r0[j] : 1.f;
+    o.sb = rev ? W.sb[j] : make_float2(0.f, 0.f);
+    return o;
+}
+
+__device__ __forceinline__ void dens_partner(const DevParams& P, DensI& d, float4 xj, const DensJ& o, bool own,
+                                             bool rev) {
     float rx = d.xi.x - xj.x, ry = d.xi.y - xj.y, rz = d.xi.z - xj.z;
     min_image(P, rx, ry);
     float Wv, gw[3];
     if (!adj::dens_pair<float>(P, rx, ry, rz, Wv, gw)) return;
     float wb = 0.f;
-    if (own) wb = d.sbi.x + d.sbi.y * adj::Fn<float>::rcp(fmaxf(T.r0[j], 1e-3f));
+    if (own) wb = d.sbi.x + d.sbi.y * adj::Fn<float>::rcp(fmaxf(o.r0, 1e-3f));
     if (rev) {   // (j, i): r_ji = -r_ij, the gradient flips sign and x_i enters with a minus
-        const float2 sbj = W.sb[j];
+        const float2 sbj = o.sb;
         wb += sbj.x + sbj.y * d.inv_r0i;
         d.rb += sbj.y * Wv * (-d.inv_r0i * d.inv_r0i);
     }
@@ -1262,7 +1286,7 @@ __device__ __forceinline__ void dens_global(const DevParams& P, uint32_t k0, con
         float rx, ry, rz;
         if (!support(P, d.xi, xj, rx, ry, rz)) return;
         const bool rev = rev_role(P, xj, a);
-        if (own || rev) dens_partner(P, T, W, d, j, xj, own, rev);
+        if (own || rev) dens_partner(P, d, xj, dens_operands(T, W, j, own, rev), own, rev);
     });
     dens_store(W, Sst, i, d);
 }
@@ -1322,14 +1346,24 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const float4 xi = KIND == 0 ? m.xi : d.xi;
         const Part a = part(P, xi, T.keys[il] - k0);
         int qn = 0;
+        // Two queue entries per iteration: their staged positions (and, for density, the
+        // partners' operands) are loaded before either is evaluated; the sums still take them in
+        // queue order.
         auto flush = [&]() {
-            for (int k = 0; wave_any(k < qn); ++k) {
-                if (k < qn) {
-                    const uint16_t e = sm.q[k][tid];
-                    const float4 pj = sm.p[e & QSLOT];
-                    const uint32_t j = __float_as_uint(pj.w);
-                    if (KIND == 0) mom_partner(P, T, W, m, j, pj, (e & QOWN) != 0, (e & QREV) != 0);
-                    else dens_partner(P, T, W, d, j, pj, (e & QOWN) != 0, (e & QREV) != 0);
+            for (int k = 0; wave_any(k < qn); k += 2) {
+                const bool a0 = k < qn, a1 = k + 1 < qn;   // k + 1 < QK: k is even
+                const uint16_t e0 = a0 ? sm.q[k][tid] : (uint16_t)0, e1 = a1 ? sm.q[k + 1][tid] : (uint16_t)0;
+                const float4 pj0 = sm.p[e0 & QSLOT], pj1 = sm.p[e1 & QSLOT];
+                const uint32_t j0 = __float_as_uint(pj0.w), j1 = __float_as_uint(pj1.w);
+                const bool own0 = (e0 & QOWN) != 0, rev0 = (e0 & QREV) != 0;
+                const bool own1 = (e1 & QOWN) != 0, rev1 = (e1 & QREV) != 0;
+                if (KIND == 0) {   // (the momentum partner's registers leave room for one at a time)
+                    if (a0) mom_partner(P, m, pj0, mom_operands(T, W, j0, rev0), own0);
+                    if (a1) mom_partner(P, m, pj1, mom_operands(T, W, j1, rev1), own1);
+                } else {
+                    const DensJ o0 = dens_operands(T, W, j0, own0, rev0), o1 = dens_operands(T, W, j1, own1, rev1);
+                    if (a0) dens_partner(P, d, pj0, o0, own0, rev0);
+                    if (a1) dens_partner(P, d, pj1, o1, own1, rev1);
                 }
             }
             qn = 0;
