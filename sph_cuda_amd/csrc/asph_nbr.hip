@@ -31,9 +31,10 @@
 // through the reference test r2 < 4h^2 unchanged.
 //
 // The in-support pair work is separated from the candidate test by a per-thread queue of LDS
-// slots (QK entries): the test loop appends, and when any lane of the wavefront has a full queue
-// all lanes drain theirs in order. Lanes therefore spend the expensive pair body only on
-// in-support pairs, in the reference order.
+// slots (QK entries): the test loop takes candidates two at a time and appends, and when a lane
+// of the wavefront could not take two more, all lanes drain theirs in order, several entries per
+// iteration with their operands loaded first. Lanes therefore spend the expensive pair body only
+// on in-support pairs, in the reference order.
 #include "asph_adjoint.h"
 #include "asph_kernels.h"
 
