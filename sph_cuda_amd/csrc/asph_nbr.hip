@@ -347,22 +347,24 @@ __device__ __forceinline__ void mom_epilogue(const DevParams& P, const MomArgs& 
 // ---------------------------------------------------------------------------------------------
 // Brick staging shared by both kernels.
 // ---------------------------------------------------------------------------------------------
-template <int S> struct HaloMax {
+template <int S, bool MOM> struct HaloMax {
     static constexpr int H = BS + 2 * (S + 1);
     static constexpr int HC = H * H * H;
-    // halo particles held in LDS at a time: what 80 KiB (two workgroups per CU) leaves beside
-    // the cell table and the queues, at most 4032
-    static constexpr int MAXP_FIT = ((81920 - (2 * (HC + 1) + 2 * QK * BT + 256)) / 16) / 64 * 64;
+    // halo particles held in LDS at a time: what the workgroup's share of the CU's LDS leaves
+    // beside the cell table and the queues, at most 4032. Momentum (up to 128 VGPRs: 4 waves per
+    // SIMD) takes 80 KiB, two workgroups per CU; density (fewer VGPRs) 52 KiB, three per CU.
+    static constexpr int LDS = MOM ? 81920 : 53248;
+    static constexpr int MAXP_FIT = ((LDS - (2 * (HC + 1) + 2 * QK * BT + 256)) / 16) / 64 * 64;
     static constexpr int MAXP = MAXP_FIT < 4032 ? MAXP_FIT : 4032;
 };
 
 template <int S, bool MOM> struct Smem {
-    float4 p[HaloMax<S>::MAXP];        // x, y, z, and 1/max(rho_old,1e-3) (density) or the
+    float4 p[HaloMax<S, MOM>::MAXP];   // x, y, z, and 1/max(rho_old,1e-3) (density) or the
                                        // particle's global index (momentum)
-    uint16_t off[HaloMax<S>::HC + 1];  // exclusive start of each halo cell over the whole halo
+    uint16_t off[HaloMax<S, MOM>::HC + 1];   // exclusive start of each halo cell over the halo
     uint16_t q[QK][BT];                // per-thread queue of LDS slots
     uint32_t wsum[BW];
-    uint8_t slab[HaloMax<S>::H + 1];   // slab boundaries in halo z-layers
+    uint8_t slab[HaloMax<S, MOM>::H + 1];    // slab boundaries in halo z-layers
     int nslab;                         // 0: the halo cannot be staged (global-memory loop)
     int modeA;                         // drift bound below 0.95 cell on every axis
     float delta[3];                    // drift bound since the last sort + rounding margin [cells]
@@ -494,7 +496,7 @@ __device__ __forceinline__ void plan(const DevParams& P, uint32_t k0, const Bric
         while (ok && z < g.Hz) {
             const uint32_t b = sm.off[z * L2];
             int z1 = z;
-            while (z1 < g.Hz && (uint32_t)sm.off[(z1 + 1) * L2] - b <= (uint32_t)HaloMax<S>::MAXP) ++z1;
+            while (z1 < g.Hz && (uint32_t)sm.off[(z1 + 1) * L2] - b <= (uint32_t)HaloMax<S, MOM>::MAXP) ++z1;
             if (z1 == z) ok = false;
             else { sm.slab[++ns] = (uint8_t)z1; z = z1; }
         }
