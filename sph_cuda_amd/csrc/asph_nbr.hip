@@ -46,10 +46,12 @@ constexpr int NB = 9;              // Morton bits per brick: 8 x 8 x 8 cells
 constexpr int BS = 8;              // brick edge in cells
 constexpr int BT = 512;            // threads per workgroup (one per brick cell)
 constexpr int BW = BT / 64;        // wavefronts per workgroup
-constexpr int QK = 12;             // queued in-support pairs per thread
+constexpr int QK = 12;             // queued in-support pairs per thread, momentum
+constexpr int QKD = 8;             // and density (its halo slab shares a smaller LDS budget)
 constexpr int DU = 2;              // queue entries per drain iteration, density
 constexpr int MU = 3;              // and momentum
-static_assert(QK % DU == 0 && QK % MU == 0 && QK % 2 == 0, "a drain iteration never reads past the queue");
+static_assert(QKD % DU == 0 && QK % MU == 0 && QK % 2 == 0, "a drain iteration never reads past the queue");
+template <bool MOM> constexpr int queue_len() { return MOM ? QK : QKD; }
 constexpr int CHUNK_MIN = 256;     // bounds of the work-item size (tuned at run time)
 constexpr int CHUNK_MAX = 1024;
 constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cells]
@@ -354,7 +356,7 @@ template <int S, bool MOM> struct HaloMax {
     // beside the cell table and the queues, at most 4032. Momentum (up to 128 VGPRs: 4 waves per
     // SIMD) takes 80 KiB, two workgroups per CU; density (fewer VGPRs) 52 KiB, three per CU.
     static constexpr int LDS = MOM ? 81920 : 53248;
-    static constexpr int MAXP_FIT = ((LDS - (2 * (HC + 1) + 2 * QK * BT + 256)) / 16) / 64 * 64;
+    static constexpr int MAXP_FIT = ((LDS - (2 * (HC + 1) + 2 * queue_len<MOM>() * BT + 256)) / 16) / 64 * 64;
     static constexpr int MAXP = MAXP_FIT < 4032 ? MAXP_FIT : 4032;
 };
 
@@ -362,7 +364,7 @@ template <int S, bool MOM> struct Smem {
     float4 p[HaloMax<S, MOM>::MAXP];   // x, y, z, and 1/max(rho_old,1e-3) (density) or the
                                        // particle's global index (momentum)
     uint16_t off[HaloMax<S, MOM>::HC + 1];   // exclusive start of each halo cell over the halo
-    uint16_t q[QK][BT];                // per-thread queue of LDS slots
+    uint16_t q[queue_len<MOM>()][BT];  // per-thread queue of LDS slots
     uint32_t wsum[BW];
     uint8_t slab[HaloMax<S, MOM>::H + 1];    // slab boundaries in halo z-layers
     int nslab;                         // 0: the halo cannot be staged (global-memory loop)
@@ -639,7 +641,7 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
                 bool act[DU];
                 float4 pj[DU];
 #pragma unroll
-                for (int u = 0; u < DU; ++u) {   // k + u < QK: k is a multiple of DU
+                for (int u = 0; u < DU; ++u) {   // k + u < QKD: k is a multiple of DU
                     act[u] = k + u < qn;
                     pj[u] = sm.p[act[u] ? sm.q[k + u][tid] : 0];
                 }
@@ -677,7 +679,7 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
                         sm.q[qn][tid] = (uint16_t)(p + 1);
                         ++qn;
                     }
-                    if (wave_any(qn > QK - 2)) flush();
+                    if (wave_any(qn > QKD - 2)) flush();
                 });
             }
             flush();
