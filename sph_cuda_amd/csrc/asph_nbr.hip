@@ -356,7 +356,8 @@ template <int S, bool MOM> struct HaloMax {
     // beside the cell table and the queues, at most 4032. Momentum (up to 128 VGPRs: 4 waves per
     // SIMD) takes 80 KiB, two workgroups per CU; density (fewer VGPRs) 52 KiB, three per CU.
     static constexpr int LDS = MOM ? 81920 : 53248;
-    static constexpr int MAXP_FIT = ((LDS - (2 * (HC + 1) + 2 * queue_len<MOM>() * BT + 256)) / 16) / 64 * 64;
+    static constexpr int MAXP_FIT =
+        ((LDS - (2 * (HC + 1) + 2 * queue_len<MOM>() * BT + 12 * H + 256)) / 16) / 64 * 64;
     static constexpr int MAXP = MAXP_FIT < 4032 ? MAXP_FIT : 4032;
 };
 
@@ -365,6 +366,8 @@ template <int S, bool MOM> struct Smem {
                                        // particle's global index (momentum)
     uint16_t off[HaloMax<S, MOM>::HC + 1];   // exclusive start of each halo cell over the halo
     uint16_t q[queue_len<MOM>()][BT];  // per-thread queue of LDS slots
+    uint32_t mk[3][HaloMax<S, MOM>::H];  // Morton bits of the halo's x, y, z cells (wrapped);
+                                         // z: ~0u where the layer lies outside [0, nz)
     uint32_t wsum[BW];
     uint8_t slab[HaloMax<S, MOM>::H + 1];    // slab boundaries in halo z-layers
     int nslab;                         // 0: the halo cannot be staged (global-memory loop)
@@ -376,6 +379,7 @@ struct BrickGeo {
     int bx0, by0, bz0;  // brick origin [cells]
     int hx0, hy0, hz0;  // halo origin
     int Hx, Hy, Hz;     // halo extent
+    uint32_t mhx, mhl;  // ceil(2^32 / Hx), ceil(2^32 / (Hx Hy)): exact division of cell indices
     bool wraps;         // the halo crosses a periodic face
 };
 
@@ -430,17 +434,38 @@ __device__ __forceinline__ BrickGeo brick_geo(const DevParams& P, const Smem<S, 
     g.hx0 = g.bx0 - mx;
     g.hy0 = g.by0 - my;
     g.hz0 = g.bz0 - mz;
+    g.mhx = 0xffffffffu / (uint32_t)g.Hx + 1u;
+    g.mhl = 0xffffffffu / (uint32_t)(g.Hx * g.Hy) + 1u;
     g.wraps = (g.hx0 < 0) || (g.hx0 + g.Hx > P.nx) || (g.hy0 < 0) || (g.hy0 + g.Hy > P.ny);
     return g;
 }
 
-__device__ __forceinline__ uint32_t halo_key(const DevParams& P, uint32_t k0, const BrickGeo& g, int c,
+// Key of halo cell c (x fastest, then y, then z). c and the divisors stay below 2^16, where
+// floor(c * ceil(2^32 / d) / 2^32) = floor(c / d) exactly.
+template <int S, bool MOM>
+__device__ __forceinline__ uint32_t halo_key(uint32_t k0, const BrickGeo& g, const Smem<S, MOM>& sm, int c,
                                              bool& inside) {
-    const int hx = c % g.Hx, hy = (c / g.Hx) % g.Hy, hz = c / (g.Hx * g.Hy);
-    const int gz = g.hz0 + hz;
-    inside = gz >= 0 && gz < P.nz;
-    return k0 | morton3((uint32_t)wrapi(g.hx0 + hx, P.nx), (uint32_t)wrapi(g.hy0 + hy, P.ny),
-                        (uint32_t)(inside ? gz : 0));
+    const uint32_t hz = __umulhi((uint32_t)c, g.mhl);
+    const uint32_t rxy = (uint32_t)c - hz * (uint32_t)(g.Hx * g.Hy);
+    const uint32_t hy = __umulhi(rxy, g.mhx);
+    const uint32_t hx = rxy - hy * (uint32_t)g.Hx;
+    const uint32_t kz = sm.mk[2][hz];
+    inside = kz != ~0u;
+    return k0 | sm.mk[0][hx] | sm.mk[1][hy] | (inside ? kz : 0u);
+}
+
+// Fills the Morton tables of the halo's cell coordinates (x and y wrapped periodically).
+template <int S, bool MOM>
+__device__ __forceinline__ void halo_tables(const DevParams& P, const BrickGeo& g, Smem<S, MOM>& sm) {
+    static_assert(BT >= 192 && HaloMax<S, MOM>::H <= 64, "one thread per table entry");
+    const int t = threadIdx.x & 63, a = threadIdx.x >> 6;
+    if (a == 0 && t < g.Hx) sm.mk[0][t] = part1by2((uint32_t)wrapi(g.hx0 + t, P.nx));
+    if (a == 1 && t < g.Hy) sm.mk[1][t] = part1by2((uint32_t)wrapi(g.hy0 + t, P.ny)) << 1;
+    if (a == 2 && t < g.Hz) {
+        const int gz = g.hz0 + t;
+        sm.mk[2][t] = (gz >= 0 && gz < P.nz) ? part1by2((uint32_t)gz) << 2 : ~0u;
+    }
+    __syncthreads();
 }
 
 // Counts the halo's particles per cell, turns the counts into start offsets over the whole
@@ -449,6 +474,7 @@ __device__ __forceinline__ uint32_t halo_key(const DevParams& P, uint32_t k0, co
 template <int S, bool MOM>
 __device__ __forceinline__ void plan(const DevParams& P, uint32_t k0, const BrickGeo& g,
                                      const uint32_t* __restrict__ cstart, Smem<S, MOM>& sm) {
+    halo_tables<S, MOM>(P, g, sm);
     const int tid = threadIdx.x;
     const int HC = g.Hx * g.Hy * g.Hz;
     const int PER = (HC + BT - 1) / BT;
@@ -458,7 +484,7 @@ __device__ __forceinline__ void plan(const DevParams& P, uint32_t k0, const Bric
         const int c = c0 + k;
         if (c >= HC) break;
         bool inside;
-        const uint32_t key = halo_key(P, k0, g, c, inside);
+        const uint32_t key = halo_key(k0, g, sm, c, inside);
         uint32_t n = inside ? cstart[key + 1] - cstart[key] : 0u;
         sm.off[c] = (uint16_t)(n < 65535u ? n : 65535u);
         local += n;
@@ -521,7 +547,7 @@ __device__ __forceinline__ void stage_slab(const DevParams& P, uint32_t k0, cons
         const uint32_t o = sm.off[c], n = (uint32_t)sm.off[c + 1] - o;
         if (n == 0) continue;
         bool inside;
-        const uint32_t key = halo_key(P, k0, g, c, inside);
+        const uint32_t key = halo_key(k0, g, sm, c, inside);
         const uint32_t st = cstart[key];
         for (uint32_t m = 0; m < n; ++m) rec(sm, o - base + m, st + m, pos[st + m]);
     }
