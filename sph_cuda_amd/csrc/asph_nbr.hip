@@ -737,6 +737,19 @@ __device__ __forceinline__ void item_range(const uint32_t* __restrict__ cstart, 
     o1 = min(b1, o0 + q);
 }
 
+// Work item taken by workgroup b of n. Workgroups go to the 8 XCDs in turn (b mod 8), each XCD
+// with its own L2; within every group of 8 x XCD_RUN workgroups, XCD x takes the XCD_RUN
+// consecutive items x * XCD_RUN + [0, XCD_RUN), bricks that are neighbours in key order and
+// share most of their halos. The list order (heaviest first) holds from group to group. A
+// bijection on [0, n): the last, partial group keeps the identity.
+constexpr uint32_t XCD_RUN = 32;
+__device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
+    constexpr uint32_t G = 8 * XCD_RUN;
+    if (b >= n / G * G) return b;
+    const uint32_t j = b % G;
+    return b - j + (j % 8) * XCD_RUN + j / 8;
+}
+
 // Workgroup b plays up to two roles: mode B for brick b (members not in mode A), and mode A for
 // work item b (members in mode A). Every particle is computed by exactly one of them.
 template <int S>
@@ -778,8 +791,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             }
         }
     }
-    if ((uint32_t)b >= *nitems) return;
-    const int2 it = items[b];
+    const uint32_t nit = *nitems;
+    if ((uint32_t)b >= nit) return;
+    const int2 it = items[xcd_item((uint32_t)b, nit)];
     const BlockMap bm = bmap[it.x];
     const MemberDesc& MD = mem[bm.m];
     const DevParams& P = MD.P;
@@ -967,8 +981,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (A.mode) reduce_maxv(A.maxv + bm.m * 192, mx, my, mz);
         }
     }
-    if ((uint32_t)b >= *nitems) return;
-    const int2 it = items[b];
+    const uint32_t nit = *nitems;
+    if ((uint32_t)b >= nit) return;
+    const int2 it = items[xcd_item((uint32_t)b, nit)];
     const BlockMap bm = bmap[it.x];
     const MemberDesc& MD = mem[bm.m];
     const DevParams& P = MD.P;
@@ -1349,8 +1364,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         }
         return;
     }
-    if ((uint32_t)b >= *nitems) return;
-    const int2 it = items[b];
+    const uint32_t nit = *nitems;
+    if ((uint32_t)b >= nit) return;
+    const int2 it = items[xcd_item((uint32_t)b, nit)];
     const uint32_t kb = (uint32_t)bmap[it.x].base;
     uint32_t o0, o1;
     item_range(T.cstart, k0, kb, it.y, (uint32_t)csize, o0, o1);
