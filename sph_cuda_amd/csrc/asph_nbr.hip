@@ -666,10 +666,15 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
             for (int k = 0; wave_any(k < qn); k += DU) {
                 bool act[DU];
                 float4 pj[DU];
+                uint32_t sl[DU];
+                // the slots are read unconditionally (k + u < QKD: k is a multiple of DU), so
+                // both reads are in flight together; a stale entry is replaced by slot 0
 #pragma unroll
-                for (int u = 0; u < DU; ++u) {   // k + u < QKD: k is a multiple of DU
+                for (int u = 0; u < DU; ++u) sl[u] = sm.q[k + u][tid];
+#pragma unroll
+                for (int u = 0; u < DU; ++u) {
                     act[u] = k + u < qn;
-                    pj[u] = sm.p[act[u] ? sm.q[k + u][tid] : 0];
+                    pj[u] = sm.p[act[u] ? sl[u] : 0u];
                 }
 #pragma unroll
                 for (int u = 0; u < DU; ++u) {
@@ -871,10 +876,15 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
                 bool act[MU];
                 float4 pj[MU], vj[MU];   // vj = (vx, vy, vz, p'_j)
                 float2 rj[MU];           // (max(rho_j, 1e-3), its reciprocal)
+                uint32_t sl[MU];
+                // slots read unconditionally (k + u < QK: k is a multiple of MU), as in the
+                // density kernel
 #pragma unroll
-                for (int u = 0; u < MU; ++u) {   // k + u < QK: k is a multiple of MU
+                for (int u = 0; u < MU; ++u) sl[u] = sm.q[k + u][tid];
+#pragma unroll
+                for (int u = 0; u < MU; ++u) {
                     act[u] = k + u < qn;
-                    pj[u] = sm.p[act[u] ? sm.q[k + u][tid] : 0];
+                    pj[u] = sm.p[act[u] ? sl[u] : 0u];
                     vj[u] = A.vel[__float_as_uint(pj[u].w)];
                     rj[u] = A.rinv[__float_as_uint(pj[u].w)];
                 }
