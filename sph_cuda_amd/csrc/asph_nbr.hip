@@ -702,14 +702,12 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
                     const float r20 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
                     diff(pj1, dxm, dym, dzm);
                     const float r21 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
-                    if (r20 < P.lim) {
-                        sm.q[qn][tid] = (uint16_t)p;
-                        ++qn;
-                    }
-                    if (two && r21 < P.lim) {
-                        sm.q[qn][tid] = (uint16_t)(p + 1);
-                        ++qn;
-                    }
+                    // both slots are written, and the count takes the accepted ones: a rejected
+                    // slot lies past the count and is overwritten (qn <= QKD - 2 here)
+                    sm.q[qn][tid] = (uint16_t)p;
+                    qn += r20 < P.lim ? 1 : 0;
+                    sm.q[qn][tid] = (uint16_t)(p + 1);
+                    qn += two && r21 < P.lim ? 1 : 0;
                     if (wave_any(qn > QKD - 2)) flush();
                 });
             }
@@ -917,14 +915,10 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
                     diff(pj1, rx, ry, rz);
                     dxt = rx * P.inv_dx; dyt = ry * P.inv_dy; dzt = rz * P.inv_dz;
                     const float r21 = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
-                    if (r20 < P.lim && r20 >= 1e-12f) {
-                        sm.q[qn][tid] = (uint16_t)p;
-                        ++qn;
-                    }
-                    if (two && r21 < P.lim && r21 >= 1e-12f) {
-                        sm.q[qn][tid] = (uint16_t)(p + 1);
-                        ++qn;
-                    }
+                    sm.q[qn][tid] = (uint16_t)p;   // as in the density kernel (qn <= QK - 2 here)
+                    qn += r20 < P.lim && r20 >= 1e-12f ? 1 : 0;
+                    sm.q[qn][tid] = (uint16_t)(p + 1);
+                    qn += two && r21 < P.lim && r21 >= 1e-12f ? 1 : 0;
                     if (wave_any(qn > QK - 2)) flush();
                 });
             }
