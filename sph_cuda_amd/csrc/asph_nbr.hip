@@ -707,7 +707,7 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
                     sm.q[qn][tid] = (uint16_t)p;
                     qn += r20 < P.lim ? 1 : 0;
                     sm.q[qn][tid] = (uint16_t)(p + 1);
-                    qn += two && r21 < P.lim ? 1 : 0;
+                    qn += (two & (r21 < P.lim)) ? 1 : 0;   // & keeps both tests branch-free
                     if (wave_any(qn > QKD - 2)) flush();
                 });
             }
@@ -916,9 +916,9 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
                     dxt = rx * P.inv_dx; dyt = ry * P.inv_dy; dzt = rz * P.inv_dz;
                     const float r21 = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
                     sm.q[qn][tid] = (uint16_t)p;   // as in the density kernel (qn <= QK - 2 here)
-                    qn += r20 < P.lim && r20 >= 1e-12f ? 1 : 0;
+                    qn += ((r20 < P.lim) & (r20 >= 1e-12f)) ? 1 : 0;
                     sm.q[qn][tid] = (uint16_t)(p + 1);
-                    qn += two && r21 < P.lim && r21 >= 1e-12f ? 1 : 0;
+                    qn += (two & (r21 < P.lim) & (r21 >= 1e-12f)) ? 1 : 0;
                     if (wave_any(qn > QK - 2)) flush();
                 });
             }
