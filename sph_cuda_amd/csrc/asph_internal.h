@@ -128,6 +128,21 @@ ASPH_DH float kgrad(const DevParams& P, float q) {
     float t = 2.0f - q;
     return ((P.kout * t) * t) / q;
 }
+// Branch-free forms for the pair loops: both pieces are evaluated with the operations above and
+// the applicable one is selected, so the values equal fker / kgrad (q > 0 for every pair whose
+// gradient is used; a placeholder's quotient is discarded).
+ASPH_DH float fker_sel(float q) {
+    const float inner = __builtin_fmaf(q * q, __builtin_fmaf(0.75f, q, -1.5f), 1.0f);
+    const float t = 2.0f - q;
+    const float outer = (0.25f * t) * (t * t);
+    return q >= 2.0f ? 0.0f : (q < 1.0f ? inner : outer);
+}
+ASPH_DH float kgrad_sel(const DevParams& P, float q) {
+    const float inner = __builtin_fmaf(2.25f, q, -3.0f) * P.kin;
+    const float t = 2.0f - q;
+    const float outer = ((P.kout * t) * t) / q;
+    return q >= 2.0f ? 0.0f : (q < 1.0f ? inner : outer);
+}
 
 // --- Morton keys, asph_h100.cu:176-186 ------------------------------------------------------------
 ASPH_DH uint32_t part1by2(uint32_t n) {

@@ -134,19 +134,18 @@ __device__ __forceinline__ void mom_pair(const DevParams& P, float rx, float ry,
                                          float inv_rhoj, float4 vj, float prj, MomAcc& A,
                                          bool act = true) {
     float q = sqrt_rn(r2t) * P.inv_h;
-    float kk = kgrad(P, q);
+    float kk = kgrad_sel(P, q);
     float gx = (kk * dxt) * P.inv_dx;
     float gy = (kk * dyt) * P.inv_dy;
     float gz = (kk * dzt) * P.inv_dz;
     float dp = (prj - pri) * (inv_rhoi * inv_rhoj);
     float vdx = vi.x - vj.x, vdy = vi.y - vj.y, vdz = vi.z - vj.z;
     float vdotr = __builtin_fmaf(vdz, rz, __builtin_fmaf(vdy, ry, vdx * rx));
-    float visc = 0.f;
-    if (vdotr < 0.f) {
-        float rp2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
-        float rbar = 0.5f * (rhoi + rhoj);
-        visc = (P.visc_c * vdotr) / (((rp2 + P.eta2) * (rbar * rbar)) * rhoj);
-    }
+    // evaluated for every pair and kept for approaching ones (vdotr < 0); the divisor is positive
+    const float rp2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
+    const float rbar = 0.5f * (rhoi + rhoj);
+    const float vq = (P.visc_c * vdotr) / (((rp2 + P.eta2) * (rbar * rbar)) * rhoj);
+    float visc = vdotr < 0.f ? vq : 0.f;
     float br = dp + visc;
     // act = false: a placeholder pair of an idle queue lane, evaluated and dropped. Its factors
     // are replaced by zeros, so the sums take fma(-0, 0, a) = a (a sum that starts at +0 never
@@ -682,7 +681,7 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
                     float dxm, dym, dzm;
                     diff(pj[u], dxm, dym, dzm);
                     float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
-                    float W = fker(sqrt_rn(r2) * P.inv_h);
+                    float W = fker_sel(sqrt_rn(r2) * P.inv_h);
                     S1 = act[u] ? S1 + W : S1;
                     S2 = act[u] ? S2 + W * pj[u].w : S2;
                 }
