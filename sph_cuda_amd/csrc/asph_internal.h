@@ -129,13 +129,15 @@ ASPH_DH float kgrad(const DevParams& P, float q) {
     return ((P.kout * t) * t) / q;
 }
 // Branch-free forms for the pair loops: both pieces are evaluated with the operations above and
-// the applicable one is selected, so the values equal fker / kgrad (q > 0 for every pair whose
-// gradient is used; a placeholder's quotient is discarded).
+// the applicable one is selected, so the values equal fker / kgrad. In fker_sel, t = max(2 - q, 0)
+// is 2 - q exactly inside the support and 0 beyond it, where the outer piece is +0. kgrad_sel
+// keeps the explicit test (the compiler then skips its division where no lane needs it, which
+// measured faster than a clamped t). A quotient at q = 0 (a placeholder pair) is never selected.
 ASPH_DH float fker_sel(float q) {
     const float inner = __builtin_fmaf(q * q, __builtin_fmaf(0.75f, q, -1.5f), 1.0f);
-    const float t = 2.0f - q;
+    const float t = fmaxf(2.0f - q, 0.0f);
     const float outer = (0.25f * t) * (t * t);
-    return q >= 2.0f ? 0.0f : (q < 1.0f ? inner : outer);
+    return q < 1.0f ? inner : outer;
 }
 ASPH_DH float kgrad_sel(const DevParams& P, float q) {
     const float inner = __builtin_fmaf(2.25f, q, -3.0f) * P.kin;
