@@ -48,7 +48,7 @@ constexpr int BT = 512;            // threads per workgroup (one per brick cell)
 constexpr int BW = BT / 64;        // wavefronts per workgroup
 constexpr int QK = 12;             // queued in-support pairs per thread, momentum
 constexpr int QKD = 6;             // and density (its halo slab shares a smaller LDS budget)
-constexpr int DU = 2;              // queue entries per drain iteration, density
+constexpr int DU = 3;              // queue entries per drain iteration, density
 constexpr int MU = 3;              // and momentum
 static_assert(QKD % DU == 0 && QK % MU == 0 && QK % 2 == 0, "a drain iteration never reads past the queue");
 template <bool MOM> constexpr int queue_len() { return MOM ? QK : QKD; }
