@@ -38,7 +38,7 @@ B_SORT = 212.0
 # command (scaled by a 1 GiB device copy read back through the same counters), averaged over the
 # 300 timed steps of the 256^3 workload; profiles/step_256cubed_hbm_traffic.md holds the table and
 # the commands. Reported as roofline.traffic for the 256^3 workload only.
-TRAFFIC_B_PER_PARTICLE = dict(momentum_kdk=198.4, density_eos=170.5, kick_drift=88.5, rain=24.6)
+TRAFFIC_B_PER_PARTICLE = dict(momentum_kdk=198.2, density_eos=170.0, kick_drift=88.5, rain=24.6)
 
 
 def log(msg):
