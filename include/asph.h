@@ -116,7 +116,9 @@ typedef struct asph_run_stats {
 
 typedef struct asph_run_args {
     double sim_end;       /* stop when simt >= sim_end (ignored if <= 0) */
-    int64_t max_steps;    /* stop after this many steps (ignored if <= 0) */
+    int64_t max_steps;    /* stop after this many steps (ignored if <= 0); asph_run and
+                             asph_run_device reject a run with neither bound (a hook of
+                             asph_run_hooked may be the only stop instead) */
     float dt;             /* initial dt; <= 0 selects the controller's cap */
     int32_t fixed_dt;     /* 1: keep dt fixed, CFL only monitors and gates the sort */
     double dump_every;    /* sim seconds between dumps (<= 0: none) */

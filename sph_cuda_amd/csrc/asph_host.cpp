@@ -675,6 +675,8 @@ int asph_run_hooked(asph_t* h, const asph_run_args* a, int32_t report_every, asp
                     asph_run_stats* st) {
     if (!h || !a) return fail("null argument");
     if (h->n_members() > 1 && !a->fixed_dt) return fail("an ensemble runs with a fixed dt");
+    if (a->sim_end <= 0 && a->max_steps <= 0 && !hook)
+        return fail("asph_run needs sim_end or max_steps (the loop would not end)");
     const asph_config& c = h->cfgs[0];
     const float D = std::fmin(std::fmin(c.dx, c.dy), c.dz);
     const float dt_static = c.cfl * D / c.cs;                                          // :794
