@@ -312,6 +312,16 @@ int asph_column_diag(asph_t* h, float* out4);
 int asph_write_dump(asph_t* h, const char* outdir, int32_t sx, int32_t sy, int32_t sz,
                     double simt, float dt, char* path_out, size_t path_cap);
 
+/* Maximum floats (cx*cy*cz*6) of asph_coarse_dump_noalloc: 4 MiB per GPU. */
+#define ASPH_NOALLOC_MAX_FLOATS 1048576
+/* asph_coarse_dump without any device allocation (no reference counterpart: for callers that
+ * cannot allocate, such as code under HIP graph capture, tight serving loops and harnesses that
+ * forbid hipMalloc). Same arguments and results as asph_coarse_dump, bit for bit; the block means
+ * are staged in a fixed device array of ASPH_NOALLOC_MAX_FLOATS floats that each GPU holds until
+ * the process ends, and a dump larger than that is rejected (ASPH_ERR). Every call on a GPU shares
+ * that GPU's array: calls that overlap on one GPU race. Synchronises the handle's stream. */
+int asph_coarse_dump_noalloc(asph_t* h, int32_t sx, int32_t sy, int32_t sz, float* out);
+
 /* --- adjoint ------------------------------------------------------------------------------- */
 
 /* Adjoint of the rain mass with respect to the initial state and the autoconversion threshold

@@ -11,6 +11,7 @@ import os
 RH_KNOTS = 8
 
 OK, ERR, CFL_ABORT = 0, 1, 2
+NOALLOC_MAX_FLOATS = 1048576   # ASPH_NOALLOC_MAX_FLOATS
 
 F_SURFACE_FLUX = 1 << 0
 F_SPONGE = 1 << 1
@@ -173,6 +174,7 @@ SIGNATURES = {
     "asph_run_device": (C.c_int, [P, C.POINTER(RunArgs), C.c_int32, C.POINTER(RunStats)]),
     "asph_get_clock": (C.c_int, [P, DP, DP, FP]),
     "asph_coarse_dump": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, FP]),
+    "asph_coarse_dump_noalloc": (C.c_int, [P, C.c_int32, C.c_int32, C.c_int32, FP]),
     "asph_column_diag": (C.c_int, [P, FP]),
     "asph_write_dump": (C.c_int, [P, C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_double,
                                   C.c_float, C.c_char_p, C.c_size_t]),
@@ -221,6 +223,11 @@ def load():
             raise AsphError(f"{cls.__name__} layout does not match the library ({name})")
     _LIB = lib
     return lib
+
+
+def last_error() -> str:
+    msg = load().asph_last_error()
+    return msg.decode() if msg else ""
 
 
 def check(rc: int, what: str = "") -> int:
