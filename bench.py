@@ -2,17 +2,24 @@
 """Throughput of the anisotropic-SPH step on one MI355X.
 
 Workload: the 256^3 = 16,777,216-particle moist step (32 x 32 x 8 km, 4:1 anisotropic kernel
-metric, supersaturated 1-2 km layer, Gaussian-correlated theta noise, dt = 0.2 s), synthetic
-initial state from the seeded generator, resident in HBM before timing starts. A "step" is one
-KDK step of the reference loop (asph_h100.cu:958-1078): displacement-gated sort, kick + drift,
-density + EOS + warm-rain microphysics, momentum + theta update + kick + sponge, rain
-sedimentation, and the per-step CFL/sort-gating readback. dt is held fixed at 0.2 s (the CFL
-controller monitors only).
+metric, supersaturated 1-2 km layer, Gaussian-correlated theta noise), synthetic initial state
+from the seeded generator, resident in HBM before timing starts. A "step" is one KDK step of the
+reference loop (asph_h100.cu:958-1078): displacement-gated sort, kick + drift, density + EOS +
+warm-rain microphysics, momentum + theta update + kick + sponge, rain sedimentation, and the
+closed-loop CFL controller with its per-step max|v| readback, dt capped at the workload's 0.2 s.
 
-Prints one JSON line: particle-steps/s, the HBM-roofline fraction of the dominant kernel from
-HIP events on the library's stream, the CPU oracle on a bounded sample of the same workload, the
-batched ensemble against sequential runs, and the adjoint/forward time ratio of the 64^3 rain
-adjoint.
+The headline times steps W+1..W+K of the run from t = 0 (the named window in `config`). The
+`sustained` leg times the same run from t = 0 to t = 60 s simulated under the controller. The
+reference scheme does not stay near-uniform: its rigid-lid density bias clumps the particles and
+raises |w| past 100 m/s by t = 70 s, so the CFL controller cuts dt towards 0.03 s and the pair
+count per particle grows several-fold; the configured 600 s is not reachable in bounded time
+(`--timeline` records the run against simulated time).
+
+Prints one JSON line: particle-steps/s, the binding roofline of the dominant kernel (VALU issue,
+from its counted instructions and HIP-event launch times on the library's stream) with the HBM
+view beside it, the CPU oracle on a bounded sample of the same workload, the 128^3 moist workload
+over its configured 600 steps, the batched ensemble against sequential runs, and the
+adjoint/forward time ratio of the 64^3 rain adjoint with its finite-difference agreement.
 """
 from __future__ import annotations
 
@@ -34,11 +41,17 @@ B_STAGE = dict(kick_drift=48 + 48, density_eos=32 + 56, momentum_kdk=84 + 36 + 4
                rain=8.0)
 B_STEP = 384.8
 B_SORT = 212.0
-# HBM bytes per particle per launch measured with FETCH_SIZE / WRITE_SIZE counters on this bench
-# command (scaled by a 1 GiB device copy read back through the same counters), averaged over the
-# 300 timed steps of the 256^3 workload; profiles/step_256cubed_hbm_traffic.md holds the table and
-# the commands. Reported as roofline.traffic for the 256^3 workload only.
-TRAFFIC_B_PER_PARTICLE = dict(momentum_kdk=198.2, density_eos=170.0, kick_drift=88.5, rain=24.6)
+# Counted per launch in profiles/ with rocprofv3 on `bench.py --steps 20 --warmup 5` (the timed
+# steps 6-25 of the 256^3 run): SQ_INSTS_VALU (wave64 VALU instructions) and HBM traffic
+# (FETCH_SIZE x 2 + WRITE_SIZE, scaled by a device copy of known size). They describe that window;
+# the bench line names the source beside them.
+PROFILED = {
+    "window": "steps 6-25 of the 256^3 run",
+    "valu_source": "profiles/step_256cubed_valu_counters.md",
+    "traffic_source": "profiles/step_256cubed_hbm_traffic.md",
+    "valu_instr": dict(momentum_kdk=4.133e9, density_eos=2.953e9),
+    "traffic_bytes": dict(momentum_kdk=198.2 * 16777216, density_eos=170.0 * 16777216),
+}
 
 
 def log(msg):
@@ -46,8 +59,9 @@ def log(msg):
 
 
 def cpu_baseline(cfg, st, dt, steps):
-    """The FP32 oracle on the host cores: `steps` steps of the same workload."""
-    from oracle.oracle import Oracle
+    """The FP32 oracle on the host cores: `steps` steps of the same workload (the first one with
+    a re-sort), with as many OpenMP threads as the oracle runs with."""
+    from oracle.oracle import Oracle, load
 
     o = Oracle(cfg)
     o.set_state(st)
@@ -57,10 +71,41 @@ def cpu_baseline(cfg, st, dt, steps):
     for k in range(steps):
         o.step(dt, 0.2 * k, do_sort=(k == 0))
     wall = time.perf_counter() - t0
-    cores = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=cfg.n * steps / wall, unit="particle-steps/s", cores=cores, kind="port",
+    threads = int(load().oracle_threads())
+    cpus = len(os.sched_getaffinity(0))
+    return dict(value=cfg.n * steps / wall, unit="particle-steps/s", cores=threads, kind="port",
                 sample=f"{steps} steps (one with a re-sort) of the {cfg.nx}^3 workload, "
-                       f"FP32 oracle, OpenMP, {wall:.1f} s")
+                       f"FP32 oracle, OpenMP with {threads} threads on {cpus} CPUs in the "
+                       f"affinity mask, {wall:.1f} s")
+
+
+def controlled_run(sim, sim_end, report_every=0, wall_cap=0.0):
+    """The reference loop under its CFL controller from the handle's state to sim_end [s];
+    with report_every, records the run every that many steps (stopping after wall_cap s)."""
+    rows = []
+    t0 = time.perf_counter()
+    last = [t0, 0]
+
+    def hook(ev):
+        now = time.perf_counter()
+        kt = sim.kernel_times()
+        n = ev.step - last[1]
+        rows.append(dict(step=int(ev.step), simt=round(ev.simt, 3), dt=round(ev.dt, 5),
+                         ms_per_step=round((now - last[0]) / max(n, 1) * 1e3, 3), sorts=int(ev.sorts),
+                         trips=int(ev.trips), umax=[round(float(x), 2) for x in ev.umax],
+                         kernel_ms_per_launch={k: round(v[0] / max(v[1], 1), 3) for k, v in kt.items()}))
+        log(f"[bench] {json.dumps(rows[-1])}")
+        last[0], last[1] = time.perf_counter(), ev.step
+        return bool(wall_cap) and (now - t0) > wall_cap
+
+    sim.synchronize()
+    t0 = time.perf_counter()
+    if report_every:
+        rc, st = sim.run(sim_end=sim_end, hook=hook, report_every=report_every)
+    else:
+        rc, st = sim.run(sim_end=sim_end)
+    sim.synchronize()
+    return rc, st, time.perf_counter() - t0, rows
 
 
 def copy_probe(mib):
@@ -243,6 +288,7 @@ def adjoint_bench(reps=3, nsteps=60, checkpoint=10):
         del f
     total = best["forward_s"] + best["adjoint_s"]
     log(f"[bench] adjoint: value+gradient {total * 1e3:.1f} ms, plain forward {plain * 1e3:.1f} ms")
+    fd = fd_agreement(cfg, st, best["grad_theta"].astype("float64"), nsteps, dt, checkpoint)
     return {
         "workload": f"rain adjoint, {cfg.nx}x{cfg.ny}x{cfg.nz} = {cfg.n} particles, {nsteps} steps of "
                     f"dt={dt} s, checkpoint every {checkpoint}",
@@ -256,6 +302,108 @@ def adjoint_bench(reps=3, nsteps=60, checkpoint=10):
         "segments": int(best["segments"]),
         "J_kg": best["J"],
         "dJ_dqc0": best["dJ_dqc0"],
+        "fd_agreement": fd,
+    }
+
+
+def fd_agreement(cfg, st, g, nsteps, dt, checkpoint):
+    """This run's gradient against the committed finite-difference reference of the same window
+    (tests/golden/c4_fd.npz, written by tests/golden/make_c4_fd.py): central differences of the FP64
+    oracle at eps = 1e-3 K and 1e-5 K along 64 seeded directions, and the FP64 host adjoint's g.d.
+    Normwise relative differences over the 64 directions, and how many agree to 1e-3."""
+    import hashlib
+
+    import numpy as np
+
+    path = os.path.join(ROOT, "tests", "golden", "c4_fd.npz")
+    if not os.path.exists(path):
+        return None
+    ref = np.load(path)
+    h = hashlib.sha256()
+    for a in (st.pos, st.vel, st.thq, st.rho):
+        h.update(np.ascontiguousarray(a, np.float32).tobytes())
+    if str(ref["state_sha256"]) != h.hexdigest() or int(ref["nsteps"]) != nsteps or \
+            int(ref["checkpoint"]) != checkpoint:
+        return {"error": "fixture does not match this workload"}
+    gd = np.array([g @ np.random.default_rng(int(ref["seed0"]) + k).standard_normal(cfg.n)
+                   for k in range(len(ref["gd_host"]))])
+
+    def cmp(a, b):
+        return {"normwise": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                "median_rel": float(np.median(np.abs(a - b) / np.abs(b))),
+                "within_1e-3": int(np.sum(np.abs(a - b) <= 1e-3 * np.abs(b)))}
+
+    out = {"directions": int(len(gd)), "reference": "tests/golden/c4_fd.npz",
+           "device_vs_fd_eps1e-3": cmp(gd, ref["fd_1e3"]), "device_vs_fd_eps1e-5": cmp(gd, ref["fd_1e5"]),
+           "host_fp64_adjoint_vs_fd_eps1e-3": cmp(ref["gd_host"], ref["fd_1e3"]),
+           "host_fp64_adjoint_vs_fd_eps1e-5": cmp(ref["gd_host"], ref["fd_1e5"]),
+           "device_vs_host_fp64_adjoint": cmp(gd, ref["gd_host"])}
+    log(f"[bench] adjoint FD agreement: {json.dumps(out)}")
+    return out
+
+
+def c2_bench(steps=600):
+    """The 128^3 moist workload (preset 2, 2,097,152 particles) over its configured 600 steps
+    from t = 0 under the CFL controller (dt capped at 0.25 s), after the autotuner."""
+    import sph_cuda_amd as s
+
+    cfg, ic, dt, _ = s.preset(2)
+    st = s.ic_generate(cfg, ic)
+    sim = s.Sim(cfg)
+    sim.set_state(st)
+    sim.sort()
+    sim.force(dt)
+    tune = sim.autotune(reps=2)
+    sim.set_profiling(True)
+    sim.kernel_times()
+    sim.synchronize()
+    t0 = time.perf_counter()
+    rc, ts = sim.run(max_steps=steps)
+    sim.synchronize()
+    wall = time.perf_counter() - t0
+    if rc != 0:
+        raise SystemExit(f"128^3 run aborted rc={rc}")
+    kt = sim.kernel_times()
+    log(f"[bench] 128^3: {steps} steps in {wall:.2f} s, t = {ts.simt:.1f} s, {ts.sorts} sorts")
+    return {
+        "workload": f"plain moist 2:1 step, {cfg.nx}x{cfg.ny}x{cfg.nz} = {cfg.n} particles, dt {dt} s "
+                    f"capped, CFL controller, steps 1-{steps} from t = 0",
+        "value": cfg.n * steps / wall, "unit": "particle-steps/s", "steps": steps,
+        "ms_per_step": wall / steps * 1e3, "simt_end_s": ts.simt, "sorts": int(ts.sorts),
+        "trips": int(ts.trips), "dt_end": ts.dt, "umax_end": [float(x) for x in ts.umax],
+        "hbm_frac_step_alg": (B_STEP * cfg.n * steps + B_SORT * cfg.n * ts.sorts) / wall / 1e9 / HBM_PEAK_GBS,
+        "kernel_ms_per_launch": {k: (v[0] / max(v[1], 1)) for k, v in kt.items()},
+        "tuning": tune["best"],
+    }
+
+
+def sustained_bench(t_end=60.0):
+    """The 256^3 run from t = 0 to t_end [s] simulated under the CFL controller."""
+    import sph_cuda_amd as s
+
+    cfg, ic, dt, _ = s.preset(3)
+    st = s.ic_generate(cfg, ic)
+    sim = s.Sim(cfg)
+    sim.set_state(st)
+    sim.sort()
+    sim.force(dt)
+    tune = sim.autotune(reps=2)
+    sim.set_profiling(True)
+    sim.kernel_times()
+    rc, ts, wall, _ = controlled_run(sim, t_end)
+    if rc != 0:
+        raise SystemExit(f"sustained run aborted rc={rc}")
+    kt = sim.kernel_times()
+    n = cfg.n
+    log(f"[bench] sustained: t = 0-{ts.simt:.1f} s in {ts.steps} steps, {wall:.2f} s")
+    return {
+        "workload": f"the 256^3 run from t = 0 to t = {t_end:g} s simulated (CFL controller, dt <= {dt} s)",
+        "value": n * ts.steps / wall, "unit": "particle-steps/s", "steps": int(ts.steps),
+        "ms_per_step": wall / ts.steps * 1e3, "simt_end_s": ts.simt, "sorts": int(ts.sorts),
+        "trips": int(ts.trips), "dt_end": ts.dt, "umax_end": [float(x) for x in ts.umax],
+        "hbm_frac_step_alg": (B_STEP * n * ts.steps + B_SORT * n * ts.sorts) / wall / 1e9 / HBM_PEAK_GBS,
+        "kernel_ms_per_launch": {k: (v[0] / max(v[1], 1)) for k, v in kt.items()},
+        "tuning": tune["best"],
     }
 
 
@@ -275,7 +423,14 @@ def main():
                     help="run only the batched leg of the ensemble (profiling)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint leg")
     ap.add_argument("--no-device-loop", action="store_true", help="skip the device-controller leg")
+    ap.add_argument("--no-sustained", action="store_true", help="skip the t = 0-60 s leg")
+    ap.add_argument("--no-c2", action="store_true", help="skip the 128^3 leg")
     ap.add_argument("--adjoint-only", action="store_true", help="run only the adjoint leg (profiling)")
+    ap.add_argument("--timeline", type=float, default=0.0,
+                    help="only: run the 256^3 workload from t = 0 towards this simulated time under "
+                         "the CFL controller, recording every --timeline-every steps")
+    ap.add_argument("--timeline-every", type=int, default=150)
+    ap.add_argument("--timeline-wall", type=float, default=420.0, help="stop the timeline after this many s")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a buffer of this size 3 times (device to device) as "
                          "the byte reference for hardware traffic counters")
@@ -301,14 +456,26 @@ def main():
     t0 = time.perf_counter()
     st = s.ic_generate(cfg, ic)
     log(f"[bench] initial state generated in {time.perf_counter() - t0:.1f} s")
-
     sim = s.Sim(cfg)
     sim.set_state(st)
-    rc, ws = sim.run(max_steps=args.warmup, dt=dt, fixed_dt=True)
+
+    if args.timeline:
+        sim.sort()
+        sim.force(dt)
+        tune = sim.autotune(reps=2)
+        sim.set_profiling(True)
+        sim.kernel_times()
+        rc, ts, wall, rows = controlled_run(sim, args.timeline, args.timeline_every, args.timeline_wall)
+        print(json.dumps({"timeline": rows, "rc": rc, "steps": int(ts.steps), "simt_end_s": ts.simt,
+                          "wall_s": wall, "value": cfg.n * ts.steps / wall, "tuning": tune["best"],
+                          "stopped_by_wall_cap": ts.simt < args.timeline - 1e-6}), flush=True)
+        return
+
+    rc, ws = sim.run(max_steps=args.warmup, dt=dt)
     if rc != 0:
         raise SystemExit(f"warmup aborted rc={rc}")
     sim.synchronize()
-    log(f"[bench] warmup {args.warmup} steps done ({ws.sorts} sorts)")
+    log(f"[bench] warmup {args.warmup} steps done ({ws.sorts} sorts, t = {ws.simt:.2f} s)")
     tune = sim.autotune(reps=2)
     log(f"[bench] autotune: {tune['best']} of {tune['trials']}")
 
@@ -316,13 +483,14 @@ def main():
     sim.kernel_times()   # reset
     sim.synchronize()
     t0 = time.perf_counter()
-    rc, ts = sim.run(max_steps=args.steps, dt=dt, fixed_dt=True)
+    rc, ts = sim.run(max_steps=args.steps)
     sim.synchronize()
     wall = time.perf_counter() - t0
     if rc != 0:
         raise SystemExit(f"timed run aborted rc={rc}")
     kt = sim.kernel_times()
-    log(f"[bench] timed {args.steps} steps in {wall:.3f} s, {ts.sorts} sorts; kernels: {kt}")
+    log(f"[bench] timed {args.steps} steps in {wall:.3f} s, {ts.sorts} sorts, t = {ts.simt:.2f} s; "
+        f"kernels: {kt}")
 
     n = cfg.n
     value = n * args.steps / wall
@@ -331,11 +499,14 @@ def main():
     ms_total, cnt = kt[dom]
     avg_s = ms_total / 1e3 / max(cnt, 1)
     bytes_launch = B_STAGE[dom] * n
-    achieved = bytes_launch / avg_s / 1e9
-    traffic = None
-    if args.preset == 3 and not args.n and args.steps == 300 and args.warmup == 10:
-        traffic = TRAFFIC_B_PER_PARTICLE[dom] * n
+    hbm_achieved = bytes_launch / avg_s / 1e9
     step_alg_gbs = (B_STEP * n * args.steps + B_SORT * n * ts.sorts) / wall / 1e9
+    probe = valu_probe()
+    default_workload = args.preset == 3 and not args.n
+    instr = PROFILED["valu_instr"].get(dom) if default_workload else None
+    traffic = PROFILED["traffic_bytes"].get(dom) if default_workload else None
+    peak_valu = probe["wave_fma_per_s"] / 1e9
+    valu_achieved = instr / avg_s / 1e9 if instr else None
 
     out = {
         "metric": "particle-steps/s & % HBM peak (plain vs batched); adjoint/forward time ratio",
@@ -352,8 +523,11 @@ def main():
         "data": "synthetic (seeded initial state, no dataset)",
         "config": {
             "workload": f"plain moist anisotropic step, {cfg.nx}x{cfg.ny}x{cfg.nz} = {n} particles, "
-                        f"dx=dy={cfg.dx} m dz={cfg.dz} m, h=1.3 cells, dt={dt} s fixed",
-            "particles": n, "sorts_in_timed_steps": int(ts.sorts),
+                        f"dx=dy={cfg.dx} m dz={cfg.dz} m, h=1.3 cells, dt {dt} s capped, CFL controller",
+            "window": f"steps {args.warmup + 1}-{args.warmup + args.steps} of the run from t = 0 "
+                      f"(t = {ws.simt:.1f}-{ts.simt:.1f} s simulated)",
+            "particles": n, "sorts_in_timed_steps": int(ts.sorts), "dt_end": ts.dt,
+            "umax_end": [float(x) for x in ts.umax],
             "hbm_frac_step_alg": step_alg_gbs / HBM_PEAK_GBS,
             "step_alg_bytes_per_particle": B_STEP, "sort_alg_bytes_per_particle": B_SORT,
             "kernel_ms_per_launch": {k: (v[0] / max(v[1], 1)) for k, v in kt.items()},
@@ -362,17 +536,32 @@ def main():
             "tuning": tune["best"],
         },
         "roofline": {
-            "bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-            "alg_bytes_per_particle": B_STAGE[dom], "traffic": traffic,
-            "traffic_GBps_at_measured_time": (traffic / avg_s / 1e9) if traffic else None,
+            "bound": "valu", "kernel": dom,
+            "achieved": valu_achieved, "peak": peak_valu, "unit": "G wave64 VALU instructions/s",
+            "frac": valu_achieved / peak_valu if valu_achieved else None,
+            "peak_source": "valu_probe below: independent non-packed FP32 FMA chains, this run",
+            "valu_instr_per_launch": instr,
+            "valu_instr_source": f"{PROFILED['valu_source']} (SQ_INSTS_VALU, {PROFILED['window']})"
+                                 if instr else None,
+            "hbm": {"achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": hbm_achieved / HBM_PEAK_GBS, "alg_bytes_per_particle": B_STAGE[dom]},
+            "traffic": traffic,
+            "traffic_source": f"{PROFILED['traffic_source']} (FETCH_SIZE, WRITE_SIZE, {PROFILED['window']})"
+                              if traffic else None,
         },
         "cpu_baseline": None,
+        "sustained": None,
+        "c2": None,
         "batched": None,
         "adjoint": None,
         "device_loop": None,
+        "valu_probe": probe,
     }
     del sim
+    if not args.no_sustained and default_workload:
+        out["sustained"] = sustained_bench()
+    if not args.no_c2:
+        out["c2"] = c2_bench()
     if not args.no_ensemble:
         out["batched"] = ensemble_bench(args.ensemble_members, args.ensemble_steps)
     if not args.no_adjoint:
@@ -381,7 +570,6 @@ def main():
         out["device_loop"] = device_loop_bench()
     if args.copy_probe_mib:
         copy_probe(args.copy_probe_mib)
-    out["valu_probe"] = valu_probe()
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle sample ...")
         out["cpu_baseline"] = cpu_baseline(cfg, st, dt, args.cpu_steps)

@@ -143,7 +143,8 @@ int asph_config_reference(asph_config* cfg, int32_t nx, int32_t ny, int32_t nz);
 
 /* Workload presets: 1 = 32^3 dry bubble, 2 = 128^3 moist 2:1, 3 = 256^3 moist 4:1,
  * 4 = 64^3 adjoint window, 5 = ensemble member template (nx,ny,nz overwritten by caller).
- * Fills cfg and (if non-NULL) ic, dt and steps. */
+ * Fills cfg and (if non-NULL) ic, dt and steps. Presets 1-3 cap the closed-loop controller's dt
+ * (dt_cap) at the workload's dt, which the controller lowers when the CFL rule requires. */
 int asph_config_preset(int32_t which, asph_config* cfg, asph_ic_spec* ic, float* dt,
                        int32_t* steps);
 

@@ -6,6 +6,8 @@
 #include <cstdio>
 #include <memory>
 
+#include <omp.h>
+
 #include "asph_oracle.hpp"
 
 using namespace oracle;
@@ -44,6 +46,9 @@ void* oracle_create(const asph_config* cfg, int fp64) {
 }
 
 void oracle_destroy(void* p) { delete (Handle*)p; }
+
+// OpenMP threads the oracle's parallel loops run with (the CPU baseline's core count).
+int oracle_threads(void) { return omp_get_max_threads(); }
 
 int oracle_set_state(void* p, const float* pos4, const float* vel4, const float* thq4,
                      const float* rho) {

@@ -34,6 +34,7 @@ def load():
     sig = {
         "oracle_create": (P, [C.POINTER(Config), C.c_int]),
         "oracle_destroy": (None, [P]),
+        "oracle_threads": (C.c_int, []),
         "oracle_set_state": (C.c_int, [P, FP, FP, FP, FP]),
         "oracle_set_state_f64": (C.c_int, [P, DP, DP, DP, DP]),
         "oracle_get_state": (C.c_int, [P, FP, FP, FP, FP, FP, FP, FP]),

@@ -237,6 +237,7 @@ int asph_config_preset(int32_t which, asph_config* c, asph_ic_spec* ic, float* d
             s.bubble_dth = 2.0f; s.bubble_r = 2000.f;
             s.bubble_x = 4000.f; s.bubble_y = 4000.f; s.bubble_z = 2000.f;
             pdt = 0.5f; pst = 120;
+            c->dt_cap = pdt;   // the workload's dt; the CFL controller lowers it when required
             break;
         case 2:   // 128^3 moist, 16x16x8 km, 2:1 anisotropy
             asph_config_reference(c, 128, 128, 128);
@@ -248,6 +249,7 @@ int asph_config_preset(int32_t which, asph_config* c, asph_ic_spec* ic, float* d
             s.bubble_dth = 1.5f; s.bubble_r = 2000.f;
             s.bubble_x = 8000.f; s.bubble_y = 8000.f; s.bubble_z = 2000.f;
             pdt = 0.25f; pst = 600;
+            c->dt_cap = pdt;
             break;
         case 3:   // 256^3 moist, 32x32x8 km, 4:1 anisotropy, supersaturated 1-2 km layer
             asph_config_reference(c, 256, 256, 256);
@@ -266,6 +268,7 @@ int asph_config_preset(int32_t which, asph_config* c, asph_ic_spec* ic, float* d
             s.seed = 3; s.jitter = 0.1f;
             s.noise_sigma = 0.2f; s.noise_len = 500.f;
             pdt = 0.2f; pst = 3000;
+            c->dt_cap = pdt;
             break;
         case 4:   // 64^3 adjoint window: 8 km cube, moist, initial cloud in the bubble
             asph_config_reference(c, 64, 64, 64);
