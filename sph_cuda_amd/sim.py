@@ -138,6 +138,18 @@ class Sim:
         args = [np.ascontiguousarray(a, np.float32) for a in (st.pos, st.vel, st.thq, st.rho)]
         L.check(self.lib.asph_set_state(self.h, *[_fp(a) for a in args]), "asph_set_state")
 
+    def set_state_device(self, pos, vel, thq, rho, acc=None):
+        """Uploads a state from device tensors (torch, contiguous float32 on this GPU): pos, vel and
+        thq of shape (N, 4), rho of shape (N,), acc (N, 4) or None (then zeroed)."""
+        for name, x, shape in (("pos", pos, (self.n, 4)), ("vel", vel, (self.n, 4)), ("thq", thq, (self.n, 4)),
+                               ("rho", rho, (self.n,)), ("acc", acc, (self.n, 4))):
+            if x is None:
+                continue
+            if not (x.is_cuda and x.is_contiguous() and str(x.dtype) == "torch.float32" and tuple(x.shape) == shape):
+                raise ValueError(f"{name}: expected a contiguous float32 CUDA tensor of shape {shape}")
+        ptr = [None if x is None else C.c_void_p(x.data_ptr()) for x in (pos, vel, thq, rho, acc)]
+        L.check(self.lib.asph_set_state_device(self.h, *ptr), "asph_set_state_device")
+
     def get_state(self) -> State:
         n = self.n
         st = State(np.empty((n, 4), np.float32), np.empty((n, 4), np.float32),

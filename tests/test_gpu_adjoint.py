@@ -151,6 +151,26 @@ def test_autograd_binding(case):
     assert float(qc0.grad) == pytest.approx(3.0 * r["dJ_dqc0"], rel=1e-6)
 
 
+def test_autograd_device_inputs_and_forward_only(case):
+    """theta0 and qc0 on the GPU: J and both gradients come back on the GPU with the values of the
+    host path; without gradients the forward window alone gives the same J bit for bit."""
+    cfg, dt, st, r, _ = case
+    dev = torch.device("cuda:0")
+    theta0 = torch.tensor(st.thq[:, 0], dtype=torch.float32, device=dev, requires_grad=True)
+    qc0 = torch.tensor(float(cfg.qc_auto), dtype=torch.float64, device=dev, requires_grad=True)
+    J = rain_objective(theta0, qc0, st, cfg, NSTEPS, dt, CK)
+    assert J.device.type == "cuda" and float(J.detach()) == r["J"]
+    J.backward()
+    assert theta0.grad.device.type == "cuda" and qc0.grad.device.type == "cuda"
+    g = r["grad_theta"].astype(np.float64)
+    np.testing.assert_allclose(theta0.grad.cpu().numpy(), g, rtol=1e-4, atol=1e-6 * np.max(np.abs(g)))
+    assert float(qc0.grad) == pytest.approx(r["dJ_dqc0"], rel=1e-6)
+    with torch.no_grad():
+        J0 = rain_objective(theta0.detach(), qc0.detach(), st, cfg, NSTEPS, dt, CK)
+    J1 = rain_objective(torch.tensor(st.thq[:, 0]), float(cfg.qc_auto), st, cfg, NSTEPS, dt, CK)
+    assert float(J0) == float(J1) == r["J"]
+
+
 def test_rejects_what_it_cannot_differentiate(case):
     cfg8, ic8, dt = small_config("adj", (8, 8, 8))   # narrower than 2 * stencil + 3
     with s.Sim(cfg8) as sim:
