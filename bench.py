@@ -79,6 +79,47 @@ def cpu_baseline(cfg, st, dt, steps):
                        f"affinity mask, {wall:.1f} s")
 
 
+def cpu_baselines(c1_steps=120, c2_steps=10, adj_steps=10):
+    """The CPU oracle on the other workloads (the reference has no CPU path): the 32^3 dry bubble
+    over its 120 CFL-controlled steps (the oracle's own run loop), 10 steps of the 128^3 moist
+    workload, and the FP64 host adjoint of the 64^3 rain objective over a 10-step window (one
+    segment), each timed around its loop only."""
+    import sph_cuda_amd as s
+    from oracle.oracle import Oracle, adjoint_rain_f64, load
+
+    threads = int(load().oracle_threads())
+    out = {"cores": threads, "kind": "port", "unit": "particle-steps/s"}
+    cfg, ic, dt, _ = s.preset(1)
+    o = Oracle(cfg)
+    o.set_state(s.ic_generate(cfg, ic))
+    t0 = time.perf_counter()
+    rc, st = o.run(max_steps=c1_steps)
+    w = time.perf_counter() - t0
+    out["c1"] = {"workload": f"dry bubble {cfg.nx}^3, {c1_steps} CFL-controlled steps (oracle run loop)",
+                 "value": cfg.n * c1_steps / w, "wall_s": w, "sorts": st["sorts"]}
+    cfg, ic, dt, _ = s.preset(2)
+    o = Oracle(cfg)
+    o.set_state(s.ic_generate(cfg, ic))
+    o.sort()
+    o.force(dt)
+    t0 = time.perf_counter()
+    for k in range(c2_steps):
+        o.step(dt, k * dt, do_sort=(k == 0))
+    w = time.perf_counter() - t0
+    out["c2"] = {"workload": f"moist {cfg.nx}^3, {c2_steps} steps (one with a re-sort)",
+                 "value": cfg.n * c2_steps / w, "ms_per_step": w / c2_steps * 1e3}
+    cfg, ic, dt, _ = s.preset(4)
+    st4 = s.ic_generate(cfg, ic)
+    t0 = time.perf_counter()
+    adjoint_rain_f64(cfg, st4, adj_steps, dt, adj_steps)
+    w = time.perf_counter() - t0
+    out["adjoint_fp64"] = {"workload": f"FP64 host adjoint of the {cfg.nx}^3 rain objective, "
+                                       f"{adj_steps}-step window (forward, tape and reverse)",
+                           "wall_s": w, "ms_per_step": w / adj_steps * 1e3}
+    log(f"[bench] CPU baselines: {json.dumps(out)}")
+    return out
+
+
 def controlled_run(sim, sim_end, report_every=0, wall_cap=0.0):
     """The reference loop under its CFL controller from the handle's state to sim_end [s];
     with report_every, records the run every that many steps (stopping after wall_cap s)."""
@@ -573,6 +614,8 @@ def main():
     if not args.no_cpu_baseline:
         log("[bench] CPU oracle sample ...")
         out["cpu_baseline"] = cpu_baseline(cfg, st, dt, args.cpu_steps)
+        if default_workload:
+            out["cpu_baselines"] = cpu_baselines()
     print(json.dumps(out), flush=True)
 
 
