@@ -60,8 +60,9 @@ constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cel
 // added (an added +0 changes nothing but the sign of a zero difference, which no sum sees).
 __device__ __forceinline__ void min_image(const DevParams& P, float& rx, float& ry) {
     if (P.flags & ASPH_F_MIN_IMAGE) {
-        const float sx = (rx > P.hLx ? -P.Lx : 0.f) + (rx < -P.hLx ? P.Lx : 0.f);
-        const float sy = (ry > P.hLy ? -P.Ly : 0.f) + (ry < -P.hLy ? P.Ly : 0.f);
+        // |r| > L/2: shift by L against the sign of r (the same value as the two tests)
+        const float sx = fabsf(rx) > P.hLx ? copysignf(P.Lx, -rx) : 0.f;
+        const float sy = fabsf(ry) > P.hLy ? copysignf(P.Ly, -ry) : 0.f;
         rx += sx;
         ry += sy;
     }
@@ -147,14 +148,13 @@ __device__ __forceinline__ void mom_pair(const DevParams& P, float rx, float ry,
     const float vq = (P.visc_c * vdotr) / (((rp2 + P.eta2) * (rbar * rbar)) * rhoj);
     float visc = vdotr < 0.f ? vq : 0.f;
     float br = dp + visc;
-    // act = false: a placeholder pair of an idle queue lane, evaluated and dropped. Its factors
-    // are replaced by zeros, so the sums take fma(-0, 0, a) = a (a sum that starts at +0 never
-    // holds -0) and nothing of the placeholder, even a non-finite value, reaches them.
+    // act = false: a placeholder pair of an idle queue lane, evaluated and dropped. Its pressure
+    // and viscous factors are replaced by zeros; its gradient is finite (kgrad_sel selects a
+    // finite piece for every finite q, and the placeholder's separation is that of two staged
+    // particles), so each sum takes fma(-0, g, a) = a for every nonzero a. Only a sum that is
+    // exactly zero at that point could change the sign of its zero.
     br = act ? br : 0.f;
     visc = act ? visc : 0.f;
-    gx = act ? gx : 0.f;
-    gy = act ? gy : 0.f;
-    gz = act ? gz : 0.f;
     A.ax = __builtin_fmaf(-br, gx, A.ax);
     A.ay = __builtin_fmaf(-br, gy, A.ay);
     A.az = __builtin_fmaf(-br, gz, A.az);
