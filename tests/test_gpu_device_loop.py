@@ -80,3 +80,14 @@ def test_cfl_trip_clamp_and_abort_match_host_loop(strict):
         assert rc1 == rc2 and s1.steps == s2.steps and s1.trips == s2.trips and s1.trips >= 1
         assert rc1 == (s._lib.CFL_ABORT if strict else 0)
         assert _same_state(g.get_state(), d.get_state())
+
+
+def test_run_without_a_stop_condition_is_rejected():
+    """Neither sim_end nor max_steps: both loops return an error instead of running forever."""
+    cfg, ic, dt = small_config("ref")
+    with s.Sim(cfg) as g:
+        g.set_state(s.ic_generate(cfg, ic))
+        with pytest.raises(s.AsphError, match="sim_end or max_steps"):
+            g.run()
+        with pytest.raises(s.AsphError, match="sim_end or max_steps"):
+            g.run_device()
