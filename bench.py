@@ -252,17 +252,14 @@ def valu_probe():
             "wave_fma_per_simd_cycle_at_2p4GHz": w.value / (2.4e9 * 1024)}
 
 
-def device_loop_bench(steps=120, poll=32, reps=3):
-    """The reference loop at the 32^3 dry workload (preset 1, CFL-controlled dt): the host loop
-    (asph_run, one host synchronisation per step for max|v|) against the same loop with its
-    controller on the device (asph_run_device, a captured graph and a poll every `poll` steps),
-    each after the autotuner. Best of `reps`; the final states must agree bit for bit."""
+def _loop_pair(cfg, st, dt, steps, poll, reps, tune):
+    """Best-of-reps wall time of the host loop (asph_run, one host synchronisation per step for
+    max|v|) and the device loop (asph_run_device: a captured graph, a poll every `poll` steps) over
+    `steps` CFL-controlled steps from the same state, and whether their final states agree."""
     import numpy as np
 
     import sph_cuda_amd as s
 
-    cfg, ic, dt, _ = s.preset(1)
-    st = s.ic_generate(cfg, ic)
     walls = {"host": None, "device": None}
     finals = {}
     for name in ("host", "device"):
@@ -271,7 +268,8 @@ def device_loop_bench(steps=120, poll=32, reps=3):
                 g.set_state(st)
                 g.sort()
                 g.force(dt)
-                g.autotune(reps=2)
+                if tune:
+                    g.autotune(reps=2)
                 g.synchronize()
                 t0 = time.perf_counter()
                 if name == "host":
@@ -286,15 +284,43 @@ def device_loop_bench(steps=120, poll=32, reps=3):
                 finals[name] = g.get_state()
     same = all(np.array_equal(getattr(finals["host"], f), getattr(finals["device"], f))
                for f in ("pos", "vel", "thq", "rho"))
+    return walls, same
+
+
+def device_loop_bench(steps=120, poll=32, reps=3):
+    """The reference loop with its controller on the host against the same loop on the device:
+    at the 32^3 dry workload (preset 1, 120 CFL-controlled steps, after the autotuner), and on the
+    smallest member of the 256-member ensemble (16 x 16 x 32 = 8,192 particles, moist, 240
+    CFL-controlled steps), where the host's per-step synchronisation weighs most. Best of `reps`;
+    the final states must agree bit for bit."""
+    import sph_cuda_amd as s
+
+    cfg, ic, dt, _ = s.preset(1)
+    walls, same = _loop_pair(cfg, s.ic_generate(cfg, ic), dt, steps, poll, reps, True)
     n = cfg.n
     log(f"[bench] device loop: host {walls['host'] * 1e3:.1f} ms, device {walls['device'] * 1e3:.1f} ms, same {same}")
-    return {
+    out = {
         "workload": f"dry bubble {cfg.nx}x{cfg.ny}x{cfg.nz} = {n} particles, {steps} CFL-controlled steps",
         "host_loop_value": n * steps / walls["host"], "device_loop_value": n * steps / walls["device"],
         "unit": "particle-steps/s", "host_loop_ms_per_step": walls["host"] / steps * 1e3,
         "device_loop_ms_per_step": walls["device"] / steps * 1e3, "speedup": walls["host"] / walls["device"],
         "poll_steps": poll, "bit_identical": bool(same),
     }
+    cfgs, ics, dte, _ = s.ensemble_preset(256, 5)
+    m = min(range(len(cfgs)), key=lambda k: cfgs[k].n)
+    c, steps_m = cfgs[m], 240
+    c.dt_cap = dte
+    walls, same = _loop_pair(c, s.ic_generate(c, ics[m]), dte, steps_m, poll, reps, False)
+    log(f"[bench] device loop (small member): host {walls['host'] * 1e3:.1f} ms, "
+        f"device {walls['device'] * 1e3:.1f} ms, same {same}")
+    out["small_member"] = {
+        "workload": f"ensemble member {m}: {c.nx}x{c.ny}x{c.nz} = {c.n} particles, moist, {steps_m} "
+                    f"CFL-controlled steps (dt <= {dte} s)",
+        "host_loop_ms_per_step": walls["host"] / steps_m * 1e3,
+        "device_loop_ms_per_step": walls["device"] / steps_m * 1e3,
+        "speedup": walls["host"] / walls["device"], "bit_identical": bool(same),
+    }
+    return out
 
 
 def adjoint_bench(reps=3, nsteps=60, checkpoint=10):
