@@ -610,6 +610,9 @@ def main():
             "valu_instr_per_launch": instr,
             "valu_instr_source": f"{PROFILED['valu_source']} (SQ_INSTS_VALU, {PROFILED['window']})"
                                  if instr else None,
+            # the counts were taken over steps 6-25; a later or longer window holds more pair work
+            # per launch, for which this fraction is a lower bound
+            "valu_window_matches": (args.warmup, args.steps) == (5, 20),
             "hbm": {"achieved": hbm_achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": hbm_achieved / HBM_PEAK_GBS, "alg_bytes_per_particle": B_STAGE[dom]},
             "traffic": traffic,
