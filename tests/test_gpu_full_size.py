@@ -83,3 +83,22 @@ def test_one_step_at_full_size(which):
     assert not bad, errs
     assert np.array_equal(g.get_rain_fx()[0], o.get_rain_fx()[0])
     assert np.array_equal(g.max_vel(), o.max_vel())
+
+
+@pytest.mark.parametrize("which,steps", [(2, 24), (3, 4)])
+def test_controlled_run_at_full_size(which, steps):
+    """The reference loop under the CFL controller (displacement-gated sorts, dt capped at the
+    workload's) from t = 0 at the workload's own size: the 128^3 workload over 24 steps and the
+    256^3 workload over 4, state, rain grid and loop statistics bit for bit."""
+    cfg, dt, _, g, o = _pair(which)
+    rc_g, sg = g.run(max_steps=steps)
+    rc_o, so = o.run(max_steps=steps)
+    assert rc_g == rc_o == 0
+    assert (sg.steps, sg.sorts, sg.trips) == (so["steps"], so["sorts"], so["trips"])
+    assert sg.simt == so["simt"] and np.float32(sg.dt) == np.float32(so["dt"])
+    a, b = g.get_state(), o.get_state()
+    print(f"[case] preset {which}: {steps} steps, {sg.sorts} sorts, t = {sg.simt:.2f} s; max |diff| "
+          f"{_max_abs(a, b)}", flush=True)
+    for f in FIELDS:
+        assert np.array_equal(getattr(a, f), getattr(b, f)), f
+    assert np.array_equal(g.get_rain_fx()[0], o.get_rain_fx()[0])
