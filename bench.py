@@ -603,9 +603,15 @@ def main():
             "tuning": tune["best"],
         },
         "roofline": {
+            # VALU issue in FP32-FMA-equivalent TFLOP/s (wave64 instructions x 64 lanes x 2) on both
+            # sides: the kernel's counted instructions and the probe's FMA rate
             "bound": "valu", "kernel": dom,
-            "achieved": valu_achieved, "peak": peak_valu, "unit": "G wave64 VALU instructions/s",
+            "achieved": valu_achieved * 128 / 1e3 if valu_achieved else None,
+            "peak": peak_valu * 128 / 1e3, "unit": "TFLOP/s",
+            "unit_note": "FP32 FMA-equivalent: VALU wave64 instructions x 128 per second, against "
+                         "the probe's non-packed FMA rate (not the 157.3 TFLOP/s packed peak)",
             "frac": valu_achieved / peak_valu if valu_achieved else None,
+            "valu_wave_instr_per_s": valu_achieved * 1e9 if valu_achieved else None,
             "peak_source": "valu_probe below: independent non-packed FP32 FMA chains, this run",
             "valu_instr_per_launch": instr,
             "valu_instr_source": f"{PROFILED['valu_source']} (SQ_INSTS_VALU, {PROFILED['window']})"
