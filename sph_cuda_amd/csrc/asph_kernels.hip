@@ -324,37 +324,43 @@ __global__ __launch_bounds__(TPB) void k_audit(const MemberDesc* __restrict__ me
     if (threadIdx.x < 7) atomicAdd(&tot[threadIdx.x], red[threadIdx.x][0]);
 }
 
-// kernel_coarse_dump, asph_h100.cu:680-703, for member m (cells in the reference's order).
+// Coarse dump (kernel_coarse_dump, asph_h100.cu:680-703) for member m: one thread per block of
+// sx x sy x sz cells (blocks x fastest, then y, then z; the high-edge blocks are partial). The six
+// fields [theta, qv, qc, u, v, w] are summed over the block's cells in x-fastest order and over the
+// particles listed in each cell in listed order, the reference's order, on which the dump's bytes
+// depend; an empty block writes zeros.
 __global__ __launch_bounds__(TPB) void k_coarse_dump(const MemberDesc* __restrict__ mem, int m,
                                                      const uint32_t* __restrict__ cstart,
                                                      const float4* __restrict__ vel,
                                                      const float4* __restrict__ thq,
                                                      float* __restrict__ out, int sx, int sy, int sz,
-                                                     int cx, int cy, int cz) {
+                                                     int nbx, int nby, int nbz) {
     const MemberDesc& M = mem[m];
     const DevParams& P = M.P;
-    int idx = blockIdx.x * TPB + threadIdx.x;
-    if (idx >= cx * cy * cz) return;
-    int cxx = idx % cx, cyy = (idx / cx) % cy, czz = idx / (cx * cy);
-    float a[6] = {0, 0, 0, 0, 0, 0};
-    int n = 0;
-    int ix1 = min((cxx + 1) * sx, P.nx), iy1 = min((cyy + 1) * sy, P.ny), iz1 = min((czz + 1) * sz, P.nz);
-    for (int iz = czz * sz; iz < iz1; ++iz)
-        for (int iy = cyy * sy; iy < iy1; ++iy)
-            for (int ix = cxx * sx; ix < ix1; ++ix) {
-                uint32_t k = M.k0 | morton3((uint32_t)ix, (uint32_t)iy, (uint32_t)iz);
-                for (uint32_t j = cstart[k]; j < cstart[k + 1]; ++j) {
-                    float4 t = thq[j], v = vel[j];
-                    a[0] += t.x; a[1] += t.y; a[2] += t.z;
-                    a[3] += v.x; a[4] += v.y; a[5] += v.z;
-                    n++;
+    const int blk = blockIdx.x * TPB + threadIdx.x;
+    if (blk >= nbx * nby * nbz) return;
+    const int x0 = (blk % nbx) * sx, y0 = ((blk / nbx) % nby) * sy, z0 = (blk / (nbx * nby)) * sz;
+    const int x1 = min(x0 + sx, P.nx), y1 = min(y0 + sy, P.ny), z1 = min(z0 + sz, P.nz);
+    float sum[6] = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
+    int count = 0;
+    for (int z = z0; z < z1; ++z)
+        for (int y = y0; y < y1; ++y)
+            for (int x = x0; x < x1; ++x) {
+                const uint32_t key = M.k0 | morton3((uint32_t)x, (uint32_t)y, (uint32_t)z);
+                for (uint32_t j = cstart[key], e = cstart[key + 1]; j < e; ++j) {
+                    const float4 tq = thq[j], u = vel[j];
+                    sum[0] += tq.x; sum[1] += tq.y; sum[2] += tq.z;
+                    sum[3] += u.x; sum[4] += u.y; sum[5] += u.z;
+                    ++count;
                 }
             }
-    float inv = n ? 1.f / (float)n : 0.f;
-    for (int c = 0; c < 6; ++c) out[(size_t)idx * 6 + c] = a[c] * inv;
+    const float w = count ? 1.f / (float)count : 0.f;
+    float* dst = out + (size_t)blk * 6;
+    for (int f = 0; f < 6; ++f) dst[f] = sum[f] * w;
 }
 
-// kernel_column_diag, asph_h100.cu:704-723, for member m.
+// Column diagnostics (kernel_column_diag, asph_h100.cu:704-723) for member m: per column, the mean
+// theta and qv and the maximum qc and |w| over the particles listed in its cells, bottom to top.
 __global__ __launch_bounds__(TPB) void k_column_diag(const MemberDesc* __restrict__ mem, int m,
                                                      const uint32_t* __restrict__ cstart,
                                                      const float4* __restrict__ thq,
@@ -362,23 +368,28 @@ __global__ __launch_bounds__(TPB) void k_column_diag(const MemberDesc* __restric
                                                      float4* __restrict__ out) {
     const MemberDesc& M = mem[m];
     const DevParams& P = M.P;
-    int col = blockIdx.x * TPB + threadIdx.x;
-    if (col >= P.ncol) return;
-    int iy = col / P.nx, ix = col % P.nx;
-    float th = 0.f, qv = 0.f, qlmax = 0.f, wmax = 0.f;
-    int n = 0;
-    for (int k = 0; k < P.nz; ++k) {
-        uint32_t key = M.k0 | morton3((uint32_t)ix, (uint32_t)iy, (uint32_t)k);
-        for (uint32_t j = cstart[key]; j < cstart[key + 1]; ++j) {
-            float4 t = thq[j];
-            th += t.x; qv += t.y;
-            qlmax = fmaxf(qlmax, t.z);
-            wmax = fmaxf(wmax, fabsf(vel[j].z));
-            n++;
+    const int c = blockIdx.x * TPB + threadIdx.x;
+    if (c >= P.ncol) return;
+    const uint32_t kxy = M.k0 | morton3((uint32_t)(c % P.nx), (uint32_t)(c / P.nx), 0u);
+    float th_sum = 0.f, qv_sum = 0.f, qc_max = 0.f, w_max = 0.f;
+    int count = 0;
+    for (int z = 0; z < P.nz; ++z) {
+        const uint32_t key = kxy | (part1by2((uint32_t)z) << 2);
+        for (uint32_t j = cstart[key], e = cstart[key + 1]; j < e; ++j) {
+            const float4 tq = thq[j];
+            th_sum += tq.x;
+            qv_sum += tq.y;
+            qc_max = fmaxf(qc_max, tq.z);
+            w_max = fmaxf(w_max, fabsf(vel[j].z));
+            ++count;
         }
     }
-    if (n) { float inv = 1.f / (float)n; th *= inv; qv *= inv; }
-    out[col] = make_float4(th, qv, qlmax, wmax);
+    if (count) {
+        const float w = 1.f / (float)count;
+        th_sum *= w;
+        qv_sum *= w;
+    }
+    out[c] = make_float4(th_sum, qv_sum, qc_max, w_max);
 }
 
 inline int nblocks(int64_t n) { return (int)((n + TPB - 1) / TPB); }
