@@ -27,6 +27,7 @@ struct asph_t {
     uint32_t nkeys = 0;
     hipStream_t stream = nullptr;
     bool own_stream = false;
+    int device = -1;        // the device current when the handle was created
     int64_t N = 0, ncol = 0;
     float4 *pos = nullptr, *vel = nullptr, *thq = nullptr, *acc = nullptr;
     float4 *pos2 = nullptr, *vel2 = nullptr, *thq2 = nullptr, *acc2 = nullptr;

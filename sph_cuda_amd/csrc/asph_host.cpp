@@ -106,6 +106,7 @@ int create_impl(const asph_config* cfgs, int n, void* stream, asph_t** out) {
         if (hipStreamCreate(&h->stream) != hipSuccess) { delete h; return fail("hipStreamCreate failed"); }
         h->own_stream = true;
     }
+    if (hipGetDevice(&h->device) != hipSuccess) h->device = -1;
     const size_t N = (size_t)h->N;
     const int nblk = (int)((N + radix_tile() - 1) / radix_tile());
     int rc = ASPH_OK;

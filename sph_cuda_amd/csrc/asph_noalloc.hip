@@ -21,10 +21,14 @@ static __device__ __align__(256) float scratch[ASPH_NOALLOC_MAX_FLOATS];
 std::mutex g_mu;
 std::vector<float*> g_ptr;   // per device id: the device address of `scratch`, or null
 
-// Device address of `scratch` on the current device (looked up on first use per device).
-int scratch_ptr(float** out) {
+// Device address of `scratch` on the handle's device, which must be the current one (looked up on
+// first use per device).
+int scratch_ptr(const asph_t* h, float** out) {
     int dev = -1;
     HIPCHK(hipGetDevice(&dev));
+    if (dev != h->device)
+        return fail("coarse dump (no alloc): device %d is current, the handle belongs to device %d", dev,
+                    h->device);
     std::lock_guard<std::mutex> lk(g_mu);
     if (dev >= (int)g_ptr.size()) g_ptr.resize((size_t)dev + 1, nullptr);
     if (!g_ptr[(size_t)dev]) {
@@ -52,7 +56,7 @@ extern "C" int asph_coarse_dump_noalloc(asph_t* h, int32_t sx, int32_t sy, int32
                     "the %d of the fixed buffer; use larger strides or asph_coarse_dump",
                     (long long)cx, (long long)cy, (long long)cz, (long long)n, ASPH_NOALLOC_MAX_FLOATS);
     float* d = nullptr;
-    if (scratch_ptr(&d) != ASPH_OK) return ASPH_ERR;
+    if (scratch_ptr(h, &d) != ASPH_OK) return ASPH_ERR;
     // every block's six means are written, empty blocks as zeros: nothing needs clearing first
     launch_coarse_dump(h->L, 0, h->stream, h->cstart, h->vel, h->thq, d, sx, sy, sz, (int)cx,
                        (int)cy, (int)cz);
