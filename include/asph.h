@@ -320,7 +320,8 @@ int asph_write_dump(asph_t* h, const char* outdir, int32_t sx, int32_t sy, int32
  * forbid hipMalloc). Same arguments and results as asph_coarse_dump, bit for bit; the block means
  * are staged in a fixed device array of ASPH_NOALLOC_MAX_FLOATS floats that each GPU holds until
  * the process ends, and a dump larger than that is rejected (ASPH_ERR). Every call on a GPU shares
- * that GPU's array: calls that overlap on one GPU race. Synchronises the handle's stream. */
+ * that GPU's array: calls that overlap on one GPU race. The handle's device must be the current one
+ * (ASPH_ERR otherwise). Synchronises the handle's stream. */
 int asph_coarse_dump_noalloc(asph_t* h, int32_t sx, int32_t sy, int32_t sz, float* out);
 
 /* --- adjoint ------------------------------------------------------------------------------- */
