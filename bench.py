@@ -224,6 +224,23 @@ def ensemble_bench(n_members, steps, seed=5, sequential=True):
             exact += int(all(np.array_equal(getattr(final, f)[a:a + n], getattr(o, f))
                              for f in ("pos", "vel", "thq", "rho")))
     log(f"[bench] sequential: {wall_s:.3f} s; members bit-identical: {exact}/{n_members}")
+    # the strongest sequential form: each member alone, its steps replayed from a captured graph
+    # with no host synchronisation (after two replayed warm-up steps that capture the graphs)
+    wall_g, exact_g = 0.0, 0
+    for m, (c, st) in enumerate(zip(cfgs, states) if sequential else []):
+        with s.Sim(c) as one:
+            one.set_state(st)
+            one.run_graph(2, dt, sort_every=1)
+            one.synchronize()
+            t0 = time.perf_counter()
+            one.run_graph(steps, dt, sort_every=1)
+            one.synchronize()
+            wall_g += time.perf_counter() - t0
+            o = one.get_state()
+            a, n = ens.member_range(m)
+            exact_g += int(all(np.array_equal(getattr(final, f)[a:a + n], getattr(o, f))
+                               for f in ("pos", "vel", "thq", "rho")))
+    log(f"[bench] sequential graph replay: {wall_g:.3f} s; members bit-identical: {exact_g}/{n_members}")
     return {
         "workload": f"{n_members}-member ensemble, {ntot} particles (8,192-131,072 per member, "
                     f"250 m lattice, nz 32), moist, dt {dt} s, re-sort every step",
@@ -234,6 +251,9 @@ def ensemble_bench(n_members, steps, seed=5, sequential=True):
         "sequential_ms_per_step": wall_s / steps * 1e3 if sequential else None,
         "speedup_vs_sequential": wall_s / wall_b if sequential else None,
         "members_bit_identical": exact if sequential else None,
+        "sequential_graph_ms_per_step": wall_g / steps * 1e3 if sequential else None,
+        "speedup_vs_sequential_graph": wall_g / wall_b if sequential else None,
+        "members_bit_identical_graph": exact_g if sequential else None,
         "tuning": tune["best"],
         "hbm_frac_step_alg": (B_STEP + B_SORT) * ntot * steps / wall_b / 1e9 / HBM_PEAK_GBS,
     }
