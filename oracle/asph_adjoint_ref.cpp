@@ -33,6 +33,8 @@ asph::DevParams devparams(const asph_config& c) {
     D.flags = c.flags;
     D.dx = P.dx; D.dy = P.dy; D.dz = P.dz; D.inv_dx = P.inv_dx; D.inv_dy = P.inv_dy; D.inv_dz = P.inv_dz;
     D.Lx = P.Lx; D.Ly = P.Ly; D.Lz = P.Lz; D.hLx = P.hLx; D.hLy = P.hLy;
+    D.fnx = P.fnx; D.fny = P.fny; D.hnx = P.hnx; D.hny = P.hny;
+    D.gmx = P.gmx; D.gmy = P.gmy; D.gmz = P.gmz;
     D.inv_h = P.inv_h; D.lim = P.lim; D.kin = P.kin; D.kout = P.kout;
     D.vcell = P.vcell; D.rho0 = P.rho0; D.pmass = P.pmass; D.mm = P.mm; D.eta2 = P.eta2; D.visc_c = P.visc_c;
     D.cs = P.cs; D.cs2 = P.cs2; D.th0 = P.th0; D.n2 = P.n2; D.exA = P.exA; D.cp_rd = P.cp_rd; D.rd_cp = P.rd_cp;
@@ -102,14 +104,18 @@ bool in_stencil(const asph::DevParams& P, int ix, int iy, int iz, int cx, int cy
 }
 
 
-void sep(const asph::DevParams& P, const double* a, const double* b, double r[3]) {
-    r[0] = a[0] - b[0]; r[1] = a[1] - b[1]; r[2] = a[2] - b[2];
+// Separation a - b in cells (the forward's form: coordinates scaled to cells, then subtracted;
+// minimum image by the period in cells), in T = double or, as the device, float.
+template <class T> void csep(const asph::DevParams& P, const T* a, const T* b, T s[3]) {
+    const T inv[3] = {(T)P.inv_dx, (T)P.inv_dy, (T)P.inv_dz};
+    for (int c = 0; c < 3; ++c) s[c] = a[c] * inv[c] - b[c] * inv[c];
     if (P.flags & ASPH_F_MIN_IMAGE) {
-        const double Lx = P.Lx, Ly = P.Ly;
-        if (r[0] > 0.5 * Lx) r[0] -= Lx; else if (r[0] < -0.5 * Lx) r[0] += Lx;
-        if (r[1] > 0.5 * Ly) r[1] -= Ly; else if (r[1] < -0.5 * Ly) r[1] += Ly;
+        const T nx = (T)P.nx, ny = (T)P.ny;
+        if (s[0] > T(0.5) * nx) s[0] -= nx; else if (s[0] < -T(0.5) * nx) s[0] += nx;
+        if (s[1] > T(0.5) * ny) s[1] -= ny; else if (s[1] < -T(0.5) * ny) s[1] += ny;
     }
 }
+void sep(const asph::DevParams& P, const double* a, const double* b, double r[3]) { csep<double>(P, a, b, r); }
 
 // Cells added to the enumeration margin of for_pairs (a test setting: the gradient must not change).
 int g_extra_margin = 0;
@@ -228,11 +234,8 @@ void reverse(const asph::DevParams& P, const Params<double>& Q, const Tape& T, c
             const float rhoif = std::max((float)T.r1[i], 1e-3f), pif = (float)T.prs[i];
             for_pairs(P, T, i, [&](int64_t j, bool own, bool rev) {
                 const Cast<float, 4> xif(xi), xjf(&T.x1[4 * j]);
-                float r[3] = {xif.v[0] - xjf.v[0], xif.v[1] - xjf.v[1], xif.v[2] - xjf.v[2]};
-                if (P.flags & ASPH_F_MIN_IMAGE) {
-                    if (r[0] > P.hLx) r[0] -= P.Lx; else if (r[0] < -P.hLx) r[0] += P.Lx;
-                    if (r[1] > P.hLy) r[1] -= P.Ly; else if (r[1] < -P.hLy) r[1] += P.Ly;
-                }
+                float r[3];
+                csep<float>(P, xif.v, xjf.v, r);
                 const Cast<float, 3> vjf(&T.v1[4 * j]);
                 const float rhojf = std::max((float)T.r1[j], 1e-3f), pjf = (float)T.prs[j];
                 const float z3[3] = {0, 0, 0};
@@ -332,11 +335,8 @@ void reverse(const asph::DevParams& P, const Params<double>& Q, const Tape& T, c
             const float ir0f = 1.0f / std::max((float)T.r0[i], 1e-3f);
             for_pairs(P, T, i, [&](int64_t j, bool own, bool rev) {
                 const Cast<float, 4> xif(xi), xjf(&T.x1[4 * j]);
-                float r[3] = {xif.v[0] - xjf.v[0], xif.v[1] - xjf.v[1], xif.v[2] - xjf.v[2]};
-                if (P.flags & ASPH_F_MIN_IMAGE) {
-                    if (r[0] > P.hLx) r[0] -= P.Lx; else if (r[0] < -P.hLx) r[0] += P.Lx;
-                    if (r[1] > P.hLy) r[1] -= P.Ly; else if (r[1] < -P.hLy) r[1] += P.Ly;
-                }
+                float r[3];
+                csep<float>(P, xif.v, xjf.v, r);
                 float W, gw[3];
                 if (!asph::adj::dens_pair<float>(P, r[0], r[1], r[2], W, gw)) return;
                 float wb = 0.f;
@@ -583,9 +583,9 @@ double oracle_check_mom_pair_fused(const asph_config* cfg, int n, uint64_t seed,
         for (int k = 0; k < n; ++k) {
             const R R0 = (R)std::sqrt((double)P.lim) * (R)0.95;
             R r[3];
-            r[0] = (R)(rnd() * R0 * 0.577) * (R)P.dx;
-            r[1] = (R)(rnd() * R0 * 0.577) * (R)P.dy;
-            r[2] = (R)(rnd() * R0 * 0.577) * (R)P.dz;
+            r[0] = (R)(rnd() * R0 * 0.577);   // separations in cells
+            r[1] = (R)(rnd() * R0 * 0.577);
+            r[2] = (R)(rnd() * R0 * 0.577);
             R vi[3], vj[3], Ai[3], AVi[3], Aj[3], AVj[3];
             for (int a = 0; a < 3; ++a) {
                 vi[a] = (R)(5 * rnd()); vj[a] = (R)(5 * rnd());

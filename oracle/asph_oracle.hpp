@@ -79,7 +79,9 @@ template <class R> struct Params {
     int64_t N;
     uint32_t flags;
     R dx, dy, dz, inv_dx, inv_dy, inv_dz, Lx, Ly, Lz, hLx, hLy;
+    R fnx, fny, hnx, hny;   // periods in x and y [cells] and their halves
     R h, inv_h, lim, kin, kout;
+    R gmx, gmy, gmz;        // mm / dx, mm / dy, mm / dz: the momentum sums' factors
     R vcell, rho0, pmass, mm, eta2, visc_c, cs, cs2;
     R th0, n2, exA, cp_rd, rd_cp;
     R sponge_sigma, sponge_frac, dthdt_cap, c_h, c_e, beta_moist, v_bg, dts_skin, ramp;
@@ -102,6 +104,8 @@ template <class R> struct Params {
         inv_dx = R(1) / dx; inv_dy = R(1) / dy; inv_dz = R(1) / dz;   // asph_h100.cu:49
         Lx = R(nx) * dx; Ly = R(ny) * dy; Lz = R(nz) * dz;            // :792
         hLx = R(0.5) * Lx; hLy = R(0.5) * Ly;
+        fnx = R(nx); fny = R(ny);
+        hnx = R(0.5) * fnx; hny = R(0.5) * fny;
         inv_h = R(1) / h;
         lim = (R(4) * h) * h;                                          // :319, :457
         R ih5 = (((inv_h * inv_h) * inv_h) * inv_h) * inv_h;
@@ -113,6 +117,7 @@ template <class R> struct Params {
         rho0 = P0 / (R_D * th0);                                       // :75
         pmass = rho0 * vcell;                                          // :76
         mm = pmass / vcell;                                            // :497
+        gmx = mm * inv_dx; gmy = mm * inv_dy; gmz = mm * inv_dz;       // canonical :463-465, :497
         R hav = R(c.h_av);
         eta2 = (R(0.01) * hav) * (R(0.01) * hav);                      // :482
         visc_c = (-R(c.alpha_av) * cs) * hav;                          // :486
@@ -369,11 +374,17 @@ template <class R> struct Oracle {
         z0 = interior ? iz - S : std::max(iz - S, 0);
         z1 = interior ? iz + S : std::min(iz + S, P.nz - 1);
     }
-    void sep(const R* pi, const R* pj, R& rx, R& ry, R& rz) const {
-        rx = pi[0] - pj[0]; ry = pi[1] - pj[1]; rz = pi[2] - pj[2];
+    // Separation x_i - x_j in cells (asph_h100.cu:315-318, 453-456 scaled by 1/dx, 1/dy, 1/dz).
+    // canonical: each coordinate is scaled to cells first, (x_i / dx) - (x_j / dx) for
+    // (x_i - x_j) / dx, and the minimum image (off in the reference) shifts by the period in
+    // cells, nx for Lx.
+    void csep(const R* pi, const R* pj, R& sx, R& sy, R& sz) const {
+        sx = pi[0] * P.inv_dx - pj[0] * P.inv_dx;
+        sy = pi[1] * P.inv_dy - pj[1] * P.inv_dy;
+        sz = pi[2] * P.inv_dz - pj[2] * P.inv_dz;
         if (flag(ASPH_F_MIN_IMAGE)) {
-            if (rx > P.hLx) rx -= P.Lx; else if (rx < -P.hLx) rx += P.Lx;
-            if (ry > P.hLy) ry -= P.Ly; else if (ry < -P.hLy) ry += P.Ly;
+            if (sx > P.hnx) sx -= P.fnx; else if (sx < -P.hnx) sx += P.fnx;
+            if (sy > P.hny) sy -= P.fny; else if (sy < -P.hny) sy += P.fny;
         }
     }
 
@@ -394,9 +405,8 @@ template <class R> struct Oracle {
                         int64_t c = row + Params<R>::wrap(jx, P.nx);
                         for (int32_t j = cell_range[2 * c]; j < cell_range[2 * c + 1]; ++j) {
                             const R* pj = &pos[4 * (int64_t)j];
-                            R rx, ry, rz;
-                            sep(pi, pj, rx, ry, rz);
-                            R dxm = rx * P.inv_dx, dym = ry * P.inv_dy, dzm = rz * P.inv_dz;
+                            R dxm, dym, dzm;
+                            csep(pi, pj, dxm, dym, dzm);
                             R r2 = mfma<R>(dzm, dzm, mfma<R>(dym, dym, dxm * dxm));
                             if (r2 < P.lim) {
                                 R W = P.fker(msqrt<R>(r2) * P.inv_h);
@@ -524,16 +534,13 @@ template <class R> struct Oracle {
                         for (int32_t j = cell_range[2 * c]; j < cell_range[2 * c + 1]; ++j) {
                             if (j == i) continue;
                             const R* pj = &pos[4 * (int64_t)j];
-                            R rx, ry, rz;
-                            sep(pi, pj, rx, ry, rz);
-                            R dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
+                            R dxt, dyt, dzt;
+                            csep(pi, pj, dxt, dyt, dzt);
                             R r2t = mfma<R>(dzt, dzt, mfma<R>(dyt, dyt, dxt * dxt));
                             if (r2t >= P.lim || r2t < R(1e-12f)) continue;            // :458
                             R q = msqrt<R>(r2t) * P.inv_h;
-                            R k = P.kgrad(q);
-                            R gx = (k * dxt) * P.inv_dx;                                // :463-465
-                            R gy = (k * dyt) * P.inv_dy;
-                            R gz = (k * dzt) * P.inv_dz;
+                            // canonical: physical separation (x_i - x_j) as cells times dx
+                            R rx = dxt * P.dx, ry = dyt * P.dy, rz = dzt * P.dz;
                             R rhoj = mmax<R>(rho[j], R(1e-3f));
                             R inv_rhoj = R(1) / rhoj;
                             R prj = prs[j];
@@ -542,22 +549,30 @@ template <class R> struct Oracle {
                             const R* vj = &vel[4 * (int64_t)j];
                             R vdx = vi[0] - vj[0], vdy = vi[1] - vj[1], vdz = vi[2] - vj[2];
                             R vdotr = mfma<R>(vdz, rz, mfma<R>(vdy, ry, vdx * rx));
+                            // canonical: the viscous denominator with rp*rp -> rp2 (:481-486) and
+                            // one division X = 1/(q D) for both quotients: dW/dq/q (:462, outer
+                            // piece) = k_out t^2 (D X) and mu/(rbar^2 rhoj) = c vdotr (q X)
+                            R rp2 = mfma<R>(rz, rz, mfma<R>(ry, ry, rx * rx));
+                            R rbar = R(0.5f) * (rhoi + rhoj);
+                            R D = ((rp2 + P.eta2) * (rbar * rbar)) * rhoj;
+                            R X = R(1) / (q * D);
+                            R t = R(2) - q;
+                            R k = q >= R(2) ? R(0)
+                                : (q < R(1) ? P.kgrad(q) : ((P.kout * t) * t) * (D * X));
                             R visc = 0;
-                            if (vdotr < R(0)) {                                          // :480-487
-                                R rp2 = mfma<R>(rz, rz, mfma<R>(ry, ry, rx * rx));
-                                R rbar = R(0.5f) * (rhoi + rhoj);
-                                // canonical: one division for mu/(rbar^2 rhoj), rp*rp -> rp2
-                                visc = (P.visc_c * vdotr) / (((rp2 + P.eta2) * (rbar * rbar)) * rhoj);
-                            }
+                            if (vdotr < R(0)) visc = (P.visc_c * vdotr) * (q * X);        // :480-487
                             R br = dp + visc;
-                            ax = mfma<R>(-br, gx, ax); ay = mfma<R>(-br, gy, ay); az = mfma<R>(-br, gz, az);
-                            avx = mfma<R>(-visc, gx, avx); avy = mfma<R>(-visc, gy, avy);
-                            avz = mfma<R>(-visc, gz, avz);
+                            // canonical: the gradient's 1/dx (:463-465) leaves the sum for the
+                            // factor gmx = mm/dx (:497): sum of -(br k) dxt, times gmx
+                            R tb = br * k, tv = visc * k;
+                            ax = mfma<R>(-tb, dxt, ax); ay = mfma<R>(-tb, dyt, ay); az = mfma<R>(-tb, dzt, az);
+                            avx = mfma<R>(-tv, dxt, avx); avy = mfma<R>(-tv, dyt, avy);
+                            avz = mfma<R>(-tv, dzt, avz);
                         }
                     }
                 }
-            ax *= P.mm; ay *= P.mm; az *= P.mm;                                        // :497-499
-            avx *= P.mm; avy *= P.mm; avz *= P.mm;
+            ax *= P.gmx; ay *= P.gmy; az *= P.gmz;                                     // :497-499
+            avx *= P.gmx; avy *= P.gmy; avz *= P.gmz;
             if (rec_av) {
                 (*rec_av)[4 * i] = avx; (*rec_av)[4 * i + 1] = avy; (*rec_av)[4 * i + 2] = avz;
                 (*rec_av)[4 * i + 3] = R(0);

@@ -407,12 +407,13 @@ template <class R> ASPH_DH void dens_vjp(const DevParams& P, R S1, R S2, R z, R 
 }
 
 // ---- pair terms --------------------------------------------------------------------------------
-// Density kernel value W(r) = f(q) and dW/dr for r = x_owner - x_neighbour (min image applied by
-// the caller). Returns false outside the support (the forward's test r2 < 4h^2, same arithmetic).
+// Density kernel value W(r) = f(q) and dW/dr for r = x_owner - x_neighbour, given as the
+// separation in cells (dxm, dym, dzm) = (x_o / dx - x_n / dx, ...) with the minimum image applied
+// by the caller, as the forward takes it. Returns false outside the support (the forward's test
+// r2 < 4h^2, same arithmetic).
 template <class R>
-ASPH_DH bool dens_pair(const DevParams& P, R rx, R ry, R rz, R& W, R gw[3]) {
+ASPH_DH bool dens_pair(const DevParams& P, R dxm, R dym, R dzm, R& W, R gw[3]) {
     const R ix = R(P.inv_dx), iy = R(P.inv_dy), iz = R(P.inv_dz);
-    const R dxm = rx * ix, dym = ry * iy, dzm = rz * iz;
     const R r2 = Fn<R>::fma(dzm, dzm, Fn<R>::fma(dym, dym, dxm * dxm));
     if (!(r2 < R(P.lim))) return false;
     const R q = Fn<R>::sqrt(r2) * R(P.inv_h);
@@ -430,8 +431,9 @@ ASPH_DH bool dens_pair(const DevParams& P, R rx, R ry, R rz, R& W, R gw[3]) {
     return true;
 }
 
-// Momentum pair (asph_h100.cu:460-491): owner o, neighbour n, r = x_o - x_n. Given the owner's
-// adjoints of its pressure/viscous sum (Ab = d/d sum(-br g)) and of its viscous-only sum
+// Momentum pair (asph_h100.cu:460-491): owner o, neighbour n, r = x_o - x_n, given as the
+// separation in cells (sx, sy, sz) as the forward takes it (r = s times (dx, dy, dz)). Given the
+// owner's adjoints of its pressure/viscous sum (Ab = d/d sum(-br g)) and of its viscous-only sum
 // (AVb = d/d sum(-visc g)), returns the adjoints of r, v_o - v_n, rho_o, rho_n, p_o, p_n.
 // False outside the support or for coincident particles (the forward's tests, same arithmetic).
 template <class R> struct PairBar {
@@ -439,11 +441,12 @@ template <class R> struct PairBar {
 };
 
 template <class R>
-ASPH_DH bool mom_pair_vjp(const DevParams& P, R rx, R ry, R rz, const R vo[3], const R vn[3], R rho_o,
+ASPH_DH bool mom_pair_vjp(const DevParams& P, R sx, R sy, R sz, const R vo[3], const R vn[3], R rho_o,
                           R rho_n, R p_o, R p_n, const R Ab[3], const R AVb[3], PairBar<R>& b) {
     const R invd[3] = {R(P.inv_dx), R(P.inv_dy), R(P.inv_dz)};
+    const R dt3[3] = {sx, sy, sz};
+    const R rx = sx * R(P.dx), ry = sy * R(P.dy), rz = sz * R(P.dz);
     const R rv[3] = {rx, ry, rz};
-    const R dt3[3] = {rx * invd[0], ry * invd[1], rz * invd[2]};
     const R r2t = Fn<R>::fma(dt3[2], dt3[2], Fn<R>::fma(dt3[1], dt3[1], dt3[0] * dt3[0]));
     if (!(r2t < R(P.lim)) || r2t < R(1e-12f)) return false;
     const R q = Fn<R>::sqrt(r2t) * R(P.inv_h);
@@ -507,14 +510,16 @@ ASPH_DH bool mom_pair_vjp(const DevParams& P, R rx, R ry, R rz, const R vo[3], c
 // denominator's common part) is evaluated once; the terms are those of mom_pair_vjp, with the
 // second orientation's r, v_o - v_n and kernel gradient negated. A zero seed drops its
 // orientation (the terms are linear in the seeds). Adds the adjoints of x_i, v_i, rho_i and p_i.
-// False outside the support or for coincident particles.
+// r = x_i - x_j is given in cells, as for mom_pair_vjp. False outside the support or for
+// coincident particles.
 template <class R>
-ASPH_DH bool mom_pair_vjp2(const DevParams& P, R rx, R ry, R rz, const R vi[3], const R vj[3], R rho_i, R rho_j,
+ASPH_DH bool mom_pair_vjp2(const DevParams& P, R sx, R sy, R sz, const R vi[3], const R vj[3], R rho_i, R rho_j,
                            R p_i, R p_j, const R Ai[3], const R AVi[3], const R Aj[3], const R AVj[3], R xb[3],
                            R vb[3], R& rb, R& pb) {
     const R invd[3] = {R(P.inv_dx), R(P.inv_dy), R(P.inv_dz)};
+    const R dt3[3] = {sx, sy, sz};
+    const R rx = sx * R(P.dx), ry = sy * R(P.dy), rz = sz * R(P.dz);
     const R rv[3] = {rx, ry, rz};
-    const R dt3[3] = {rx * invd[0], ry * invd[1], rz * invd[2]};
     const R r2t = Fn<R>::fma(dt3[2], dt3[2], Fn<R>::fma(dt3[1], dt3[1], dt3[0] * dt3[0]));
     if (!(r2t < R(P.lim)) || r2t < R(1e-12f)) return false;
     const R q = Fn<R>::sqrt(r2t) * R(P.inv_h);

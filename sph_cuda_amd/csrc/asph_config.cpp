@@ -156,6 +156,8 @@ void make_params(const asph_config& c, DevParams& P) {
     P.inv_dx = 1.0f / c.dx; P.inv_dy = 1.0f / c.dy; P.inv_dz = 1.0f / c.dz;
     P.Lx = (float)c.nx * c.dx; P.Ly = (float)c.ny * c.dy; P.Lz = (float)c.nz * c.dz;
     P.hLx = 0.5f * P.Lx; P.hLy = 0.5f * P.Ly;
+    P.fnx = (float)c.nx; P.fny = (float)c.ny;
+    P.hnx = 0.5f * P.fnx; P.hny = 0.5f * P.fny;
     P.inv_h = 1.0f / c.h;
     P.lim = (4.0f * c.h) * c.h;
     float ih5 = (((P.inv_h * P.inv_h) * P.inv_h) * P.inv_h) * P.inv_h;
@@ -167,6 +169,7 @@ void make_params(const asph_config& c, DevParams& P) {
     P.rho0 = P0 / (R_D * c.th0);
     P.pmass = P.rho0 * P.vcell;
     P.mm = P.pmass / P.vcell;
+    P.gmx = P.mm * P.inv_dx; P.gmy = P.mm * P.inv_dy; P.gmz = P.mm * P.inv_dz;
     P.eta2 = (0.01f * c.h_av) * (0.01f * c.h_av);
     P.visc_c = (-c.alpha_av * c.cs) * c.h_av;
     P.exA = c.n2 > 0.0f ? (GRAV * GRAV) / ((CP * c.th0) * c.n2) : 0.0f;

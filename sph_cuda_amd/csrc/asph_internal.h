@@ -32,7 +32,9 @@ struct DevParams {
     int32_t N, ncell, ncol, end_bit;
     uint32_t flags;
     float dx, dy, dz, inv_dx, inv_dy, inv_dz, Lx, Ly, Lz, hLx, hLy;
+    float fnx, fny, hnx, hny;   // periods in x and y [cells] and their halves
     float inv_h, lim, kin, kout;
+    float gmx, gmy, gmz;        // mm / dx, mm / dy, mm / dz: the momentum sums' factors
     float vcell, rho0, pmass, mm, eta2, visc_c, cs, cs2;
     float th0, n2, exA, cp_rd, rd_cp;
     float sponge_sigma, sponge_frac, sponge_zs, sponge_depth, dthdt_cap;

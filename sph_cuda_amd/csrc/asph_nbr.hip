@@ -56,16 +56,37 @@ constexpr int CHUNK_MIN = 256;     // bounds of the work-item size (tuned at run
 constexpr int CHUNK_MAX = 1024;
 constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cells]
 
-// `if (r > h) r -= L; else if (r < -h) r += L;` without branches: the shift is selected and
-// added (an added +0 changes nothing but the sign of a zero difference, which no sum sees).
-__device__ __forceinline__ void min_image(const DevParams& P, float& rx, float& ry) {
+// Pair separations are taken in cells: each coordinate is scaled by 1/dx (1/dy, 1/dz) first
+// and the scaled coordinates are subtracted, (x_i / dx) - (x_j / dx). The staged halos hold the
+// scaled coordinates, so a candidate test needs no multiplication before its squares.
+__device__ __forceinline__ float4 cellpos(const DevParams& P, float4 x) {
+    return make_float4(x.x * P.inv_dx, x.y * P.inv_dy, x.z * P.inv_dz, x.w);
+}
+
+// Minimum image of a separation in cells, `if (s > n/2) s -= n; else if (s < -n/2) s += n;`
+// without branches: the shift is selected and added (an added +0 changes nothing but the sign of
+// a zero difference, which no sum sees).
+__device__ __forceinline__ void min_image(const DevParams& P, float& sx, float& sy) {
     if (P.flags & ASPH_F_MIN_IMAGE) {
-        // |r| > L/2: shift by L against the sign of r (the same value as the two tests)
-        const float sx = fabsf(rx) > P.hLx ? copysignf(P.Lx, -rx) : 0.f;
-        const float sy = fabsf(ry) > P.hLy ? copysignf(P.Ly, -ry) : 0.f;
-        rx += sx;
-        ry += sy;
+        // |s| > n/2: shift by n against the sign of s (the same value as the two tests)
+        const float tx = fabsf(sx) > P.hnx ? copysignf(P.fnx, -sx) : 0.f;
+        const float ty = fabsf(sy) > P.hny ? copysignf(P.fny, -sy) : 0.f;
+        sx += tx;
+        sy += ty;
     }
+}
+
+// Separation of i and j in cells from their scaled coordinates (minimum image when WRAP).
+template <bool WRAP = true>
+__device__ __forceinline__ void csep(const DevParams& P, float4 ci, float4 cj, float& sx, float& sy, float& sz) {
+    sx = ci.x - cj.x;
+    sy = ci.y - cj.y;
+    sz = ci.z - cj.z;
+    if (WRAP) min_image(P, sx, sy);
+}
+
+__device__ __forceinline__ float r2of(float sx, float sy, float sz) {
+    return __builtin_fmaf(sz, sz, __builtin_fmaf(sy, sy, sx * sx));
 }
 
 // Correctly rounded square root for 0 and arguments above 2^-96 (squared pair distances): the
@@ -98,6 +119,7 @@ __device__ __forceinline__ void dens_sum_global(const DevParams& P, uint32_t k0,
     int ix, iy, iz, z0, z1;
     cell_trunc(P, pi.x, pi.y, pi.z, ix, iy, iz);
     zrange<S>(P, iz, z0, z1);
+    const float4 ci = cellpos(P, pi);
     S1 = 0.f;
     S2 = 0.f;
     for (int jz = z0; jz <= z1; ++jz) {
@@ -108,11 +130,9 @@ __device__ __forceinline__ void dens_sum_global(const DevParams& P, uint32_t k0,
                 uint32_t k = kyz | part1by2((uint32_t)wrapi(jx, P.nx));
                 uint32_t s = cstart[k], e = cstart[k + 1];
                 for (uint32_t j = s; j < e; ++j) {
-                    float4 pj = pos[j];
-                    float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                    min_image(P, rx, ry);
-                    float dxm = rx * P.inv_dx, dym = ry * P.inv_dy, dzm = rz * P.inv_dz;
-                    float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+                    float dxm, dym, dzm;
+                    csep(P, ci, cellpos(P, pos[j]), dxm, dym, dzm);
+                    float r2 = r2of(dxm, dym, dzm);
                     if (r2 < P.lim) {
                         float W = fker(sqrt_rn(r2) * P.inv_h);
                         S1 += W;
@@ -128,39 +148,45 @@ struct MomAcc {
     float ax, ay, az, avx, avy, avz;
 };
 
-// One in-support momentum pair (asph_h100.cu:460-491), shared by all paths.
-__device__ __forceinline__ void mom_pair(const DevParams& P, float rx, float ry, float rz,
-                                         float dxt, float dyt, float dzt, float r2t, float4 vi,
-                                         float rhoi, float inv_rhoi, float pri, float rhoj,
+// One in-support momentum pair (asph_h100.cu:460-491), shared by all paths, from the separation
+// in cells (sx, sy, sz) and its square r2t. The physical separation is s times (dx, dy, dz). One
+// division X = 1/(q D), D the viscous denominator, gives both quotients: the outer spline piece's
+// dW/dq / q = k_out t^2 (D X) and the viscous factor c vdotr / D = c vdotr (q X). The gradient's
+// 1/dx stays out of the sums: they take -(br k) s_x, and the epilogue multiplies by mm / dx.
+__device__ __forceinline__ void mom_pair(const DevParams& P, float sx, float sy, float sz, float r2t,
+                                         float4 vi, float rhoi, float inv_rhoi, float pri, float rhoj,
                                          float inv_rhoj, float4 vj, float prj, MomAcc& A,
                                          bool act = true) {
-    float q = sqrt_rn(r2t) * P.inv_h;
-    float kk = kgrad_sel(P, q);
-    float gx = (kk * dxt) * P.inv_dx;
-    float gy = (kk * dyt) * P.inv_dy;
-    float gz = (kk * dzt) * P.inv_dz;
-    float dp = (prj - pri) * (inv_rhoi * inv_rhoj);
-    float vdx = vi.x - vj.x, vdy = vi.y - vj.y, vdz = vi.z - vj.z;
-    float vdotr = __builtin_fmaf(vdz, rz, __builtin_fmaf(vdy, ry, vdx * rx));
-    // evaluated for every pair and kept for approaching ones (vdotr < 0); the divisor is positive
+    const float q = sqrt_rn(r2t) * P.inv_h;
+    const float rx = sx * P.dx, ry = sy * P.dy, rz = sz * P.dz;
+    const float dp = (prj - pri) * (inv_rhoi * inv_rhoj);
+    const float vdx = vi.x - vj.x, vdy = vi.y - vj.y, vdz = vi.z - vj.z;
+    const float vdotr = __builtin_fmaf(vdz, rz, __builtin_fmaf(vdy, ry, vdx * rx));
     const float rp2 = __builtin_fmaf(rz, rz, __builtin_fmaf(ry, ry, rx * rx));
     const float rbar = 0.5f * (rhoi + rhoj);
-    const float vq = (P.visc_c * vdotr) / (((rp2 + P.eta2) * (rbar * rbar)) * rhoj);
-    float visc = vdotr < 0.f ? vq : 0.f;
+    const float D = ((rp2 + P.eta2) * (rbar * rbar)) * rhoj;   // positive
+    const float X = 1.0f / (q * D);
+    const float inner = __builtin_fmaf(2.25f, q, -3.0f) * P.kin;
+    const float t = 2.0f - q;
+    const float outer = ((P.kout * t) * t) * (D * X);
+    // q = 0 (a placeholder pair) makes X and the outer piece infinite: never selected
+    const float kk = q >= 2.0f ? 0.0f : (q < 1.0f ? inner : outer);
+    // evaluated for every pair and kept for approaching ones (vdotr < 0)
+    float visc = vdotr < 0.f ? (P.visc_c * vdotr) * (q * X) : 0.f;
     float br = dp + visc;
     // act = false: a placeholder pair of an idle queue lane, evaluated and dropped. Its pressure
-    // and viscous factors are replaced by zeros; its gradient is finite (kgrad_sel selects a
-    // finite piece for every finite q, and the placeholder's separation is that of two staged
-    // particles), so each sum takes fma(-0, g, a) = a for every nonzero a. Only a sum that is
-    // exactly zero at that point could change the sign of its zero.
+    // and viscous factors are replaced by zeros and its kk is finite, so each sum takes
+    // fma(-0, s, a) = a for every nonzero a. Only a sum that is exactly zero at that point could
+    // change the sign of its zero.
     br = act ? br : 0.f;
     visc = act ? visc : 0.f;
-    A.ax = __builtin_fmaf(-br, gx, A.ax);
-    A.ay = __builtin_fmaf(-br, gy, A.ay);
-    A.az = __builtin_fmaf(-br, gz, A.az);
-    A.avx = __builtin_fmaf(-visc, gx, A.avx);
-    A.avy = __builtin_fmaf(-visc, gy, A.avy);
-    A.avz = __builtin_fmaf(-visc, gz, A.avz);
+    const float tb = br * kk, tv = visc * kk;
+    A.ax = __builtin_fmaf(-tb, sx, A.ax);
+    A.ay = __builtin_fmaf(-tb, sy, A.ay);
+    A.az = __builtin_fmaf(-tb, sz, A.az);
+    A.avx = __builtin_fmaf(-tv, sx, A.avx);
+    A.avy = __builtin_fmaf(-tv, sy, A.avy);
+    A.avz = __builtin_fmaf(-tv, sz, A.avz);
 }
 
 template <int S>
@@ -171,6 +197,7 @@ __device__ __forceinline__ void mom_sum_global(const DevParams& P, uint32_t k0, 
     int ix, iy, iz, z0, z1;
     cell_trunc(P, pi.x, pi.y, pi.z, ix, iy, iz);
     zrange<S>(P, iz, z0, z1);
+    const float4 ci = cellpos(P, pi);
     const float inv_rhoi = 1.0f / rhoi;
     for (int jz = z0; jz <= z1; ++jz) {
         uint32_t kz = k0 | (part1by2((uint32_t)jz) << 2);
@@ -180,16 +207,14 @@ __device__ __forceinline__ void mom_sum_global(const DevParams& P, uint32_t k0, 
                 uint32_t k = kyz | part1by2((uint32_t)wrapi(jx, P.nx));
                 uint32_t s = cstart[k], e = cstart[k + 1];
                 for (uint32_t j = s; j < e; ++j) {
-                    float4 pj = pos[j];
-                    float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-                    min_image(P, rx, ry);
-                    float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
-                    float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
+                    float dxt, dyt, dzt;
+                    csep(P, ci, cellpos(P, pos[j]), dxt, dyt, dzt);
+                    const float r2t = r2of(dxt, dyt, dzt);
                     // j == i has r2t = 0 and is excluded here as by `if (j==i) continue` (:452)
                     if (r2t >= P.lim || r2t < 1e-12f) continue;
                     const float rhoj = fmaxf(rho[j], 1e-3f);
-                    mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, rhoj,
-                             1.0f / rhoj, vel[j], prs[j], A);
+                    mom_pair(P, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, rhoj, 1.0f / rhoj, vel[j],
+                             prs[j], A);
                 }
             }
         }
@@ -308,8 +333,8 @@ __device__ __forceinline__ void dens_epilogue(const DevParams& P, const DensArgs
 // momentum finish (:497-510) + theta update, second kick, sponge (mode 1). Returns |v| maxima.
 __device__ __forceinline__ void mom_epilogue(const DevParams& P, const MomArgs& A, int i, float4 pi, float4 vi,
                              float4 ti, float pri, MomAcc M, float& mx, float& my, float& mz) {
-    float ax = M.ax * P.mm, ay = M.ay * P.mm, az = M.az * P.mm;
-    float avx = M.avx * P.mm, avy = M.avy * P.mm, avz = M.avz * P.mm;
+    float ax = M.ax * P.gmx, ay = M.ay * P.gmy, az = M.az * P.gmz;
+    float avx = M.avx * P.gmx, avy = M.avy * P.gmy, avz = M.avz * P.gmz;
     const float z = pi.z;
     const float ex = exner_ref(P, z);
     const float thr = theta_ref(P, z);
@@ -361,8 +386,9 @@ template <int S, bool MOM> struct HaloMax {
 };
 
 template <int S, bool MOM> struct Smem {
-    float4 p[HaloMax<S, MOM>::MAXP];   // x, y, z, and 1/max(rho_old,1e-3) (density) or the
-                                       // particle's global index (momentum)
+    float4 p[HaloMax<S, MOM>::MAXP];   // x/dx, y/dy, z/dz (cells), and 1/max(rho_old,1e-3)
+                                       // (density) or the particle's global index (momentum);
+                                       // the adjoint stages x, y, z unscaled
     uint16_t off[HaloMax<S, MOM>::HC + 1];   // exclusive start of each halo cell over the halo
     uint16_t q[queue_len<MOM>()][BT];  // per-thread queue of LDS slots
     uint32_t mk[3][HaloMax<S, MOM>::H];  // Morton bits of the halo's x, y, z cells (wrapped);
@@ -650,13 +676,10 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
         const bool active = i < o1;
         const float4 pi = active ? A.pos[i] : make_float4(0.f, 0.f, 0.f, 0.f);
         const Cur cc = cur_cell<S>(P, g, pi);
+        const float4 ci = cellpos(P, pi);
         float S1 = 0.f, S2 = 0.f;
         int qn = 0, npair = 0, ncand = 0;
-        auto diff = [&](float4 pj, float& dxm, float& dym, float& dzm) {
-            float rx = pi.x - pj.x, ry = pi.y - pj.y, rz = pi.z - pj.z;
-            if (WRAP) min_image(P, rx, ry);
-            dxm = rx * P.inv_dx; dym = ry * P.inv_dy; dzm = rz * P.inv_dz;
-        };
+        auto diff = [&](float4 cj, float& dxm, float& dym, float& dzm) { csep<WRAP>(P, ci, cj, dxm, dym, dzm); };
         auto flush = [&]() {
             // lanes past their queue evaluate a placeholder (slot 0 of the staged slab, which
             // holds a particle whenever any lane has queued one) and drop it: no divergent region
@@ -680,8 +703,7 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
                     if (u > 0 && !wave_any(act[u])) break;
                     float dxm, dym, dzm;
                     diff(pj[u], dxm, dym, dzm);
-                    float r2 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
-                    float W = fker_sel(sqrt_rn(r2) * P.inv_h);
+                    float W = fker_sel(sqrt_rn(r2of(dxm, dym, dzm)) * P.inv_h);
                     S1 = act[u] ? S1 + W : S1;
                     S2 = act[u] ? S2 + W * pj[u].w : S2;
                 }
@@ -698,9 +720,9 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
                     const float4 pj0 = sm.p[p], pj1 = sm.p[two ? p + 1 : p];
                     float dxm, dym, dzm;
                     diff(pj0, dxm, dym, dzm);
-                    const float r20 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+                    const float r20 = r2of(dxm, dym, dzm);
                     diff(pj1, dxm, dym, dzm);
-                    const float r21 = __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm));
+                    const float r21 = r2of(dxm, dym, dzm);
                     // both slots are written, and the count takes the accepted ones: a rejected
                     // slot lies past the count and is overwritten (qn <= QKD - 2 here)
                     sm.q[qn][tid] = (uint16_t)p;
@@ -825,7 +847,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     count_wg(A.stats, ASPH_C_DENS_STAGINGS, sm.nslab * (int)((o1 - o0 + BT - 1) / BT));
     const float* rho_old = A.rho_old;
     auto rec = [&](Smem<S, false>& s, uint32_t slot, uint32_t jg, float4 pj) {
-        s.p[slot] = make_float4(pj.x, pj.y, pj.z, 1.0f / fmaxf(rho_old[jg], 1e-3f));
+        s.p[slot] = make_float4(pj.x * P.inv_dx, pj.y * P.inv_dy, pj.z * P.inv_dz, 1.0f / fmaxf(rho_old[jg], 1e-3f));
     };
     if (g.wraps) dens_brick<S, true>(P, A, K, k0, g, o0, o1, sm, rec);
     else dens_brick<S, false>(P, A, K, k0, g, o0, o1, sm, rec);
@@ -854,13 +876,11 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
             pri = A.prs[i];
         }
         const Cur cc = cur_cell<S>(P, g, pi);
+        const float4 ci = cellpos(P, pi);
         const float inv_rhoi = 1.0f / rhoi;
         MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
         int qn = 0, npair = 0, ncand = 0;
-        auto diff = [&](float4 pj, float& rx, float& ry, float& rz) {
-            rx = pi.x - pj.x; ry = pi.y - pj.y; rz = pi.z - pj.z;
-            if (WRAP) min_image(P, rx, ry);
-        };
+        auto diff = [&](float4 cj, float& dxt, float& dyt, float& dzt) { csep<WRAP>(P, ci, cj, dxt, dyt, dzt); };
         auto flush = [&]() {
             // lanes past their queue evaluate a placeholder (slot 0, as in the density kernel).
             // Called only when some lane has queued a pair; the loop tests at the bottom, so the
@@ -888,12 +908,10 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
 #pragma unroll
                 for (int u = 0; u < MU; ++u) {
                     if (u > 0 && !wave_any(act[u])) break;
-                    float rx, ry, rz;
-                    diff(pj[u], rx, ry, rz);
-                    float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
-                    float r2t = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
-                    mom_pair(P, rx, ry, rz, dxt, dyt, dzt, r2t, vi, rhoi, inv_rhoi, pri, rj[u].x, rj[u].y, vj[u],
-                             vj[u].w, L, act[u]);
+                    float dxt, dyt, dzt;
+                    diff(pj[u], dxt, dyt, dzt);
+                    mom_pair(P, dxt, dyt, dzt, r2of(dxt, dyt, dzt), vi, rhoi, inv_rhoi, pri, rj[u].x, rj[u].y,
+                             vj[u], vj[u].w, L, act[u]);
                 }
                 k += MU;
             } while (wave_any(k < qn));
@@ -907,13 +925,11 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
                 // candidates p and p + 1 (if two), as in the density kernel
                 ncand += walk_rows<S>(P, g, sm.delta, sm.off, sm.slab, s, cc, [&](int p, bool two) {
                     const float4 pj0 = sm.p[p], pj1 = sm.p[two ? p + 1 : p];
-                    float rx, ry, rz;
-                    diff(pj0, rx, ry, rz);
-                    float dxt = rx * P.inv_dx, dyt = ry * P.inv_dy, dzt = rz * P.inv_dz;
-                    const float r20 = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
-                    diff(pj1, rx, ry, rz);
-                    dxt = rx * P.inv_dx; dyt = ry * P.inv_dy; dzt = rz * P.inv_dz;
-                    const float r21 = __builtin_fmaf(dzt, dzt, __builtin_fmaf(dyt, dyt, dxt * dxt));
+                    float dxt, dyt, dzt;
+                    diff(pj0, dxt, dyt, dzt);
+                    const float r20 = r2of(dxt, dyt, dzt);
+                    diff(pj1, dxt, dyt, dzt);
+                    const float r21 = r2of(dxt, dyt, dzt);
                     sm.q[qn][tid] = (uint16_t)p;   // as in the density kernel (qn <= QK - 2 here)
                     qn += ((r20 < P.lim) & (r20 >= 1e-12f)) ? 1 : 0;
                     sm.q[qn][tid] = (uint16_t)(p + 1);
@@ -1012,7 +1028,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         count_wg(A.stats, sm.nslab == 1 ? ASPH_C_MOM_ONE_SLAB : ASPH_C_MOM_SLABBED);
         count_wg(A.stats, ASPH_C_MOM_STAGINGS, sm.nslab * (int)((o1 - o0 + BT - 1) / BT));
         auto rec = [&](Smem<S, true>& s, uint32_t slot, uint32_t jg, float4 pj) {
-            s.p[slot] = make_float4(pj.x, pj.y, pj.z, __uint_as_float(jg));
+            s.p[slot] = make_float4(pj.x * P.inv_dx, pj.y * P.inv_dy, pj.z * P.inv_dz, __uint_as_float(jg));
         };
         if (g.wraps) mom_brick<S, true>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
         else mom_brick<S, false>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
@@ -1039,17 +1055,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // ---------------------------------------------------------------------------------------------
 namespace adjrev {
 
-__device__ __forceinline__ int fold(int d, int n) {   // (-n, n) -> (-n/2, n/2]
-    if (d < 0) d += n;
-    if (d > n / 2) d -= n;
-    return d;
-}
-__device__ __forceinline__ int wrap1(int c, int n) { return c < 0 ? c + n : (c >= n ? c - n : c); }
-
 // Whether cell (cx, cy, cz), 0 <= cz < nz, lies in the stencil of current cell (ix, iy, iz),
 // asph_h100.cu:297-312. The z range clipped at the walls, [max(iz - S, 0), min(iz + S, nz - 1)],
-// holds such a cz exactly when |cz - iz| <= S; across a periodic axis |fold(d)| <= S holds exactly
-// when |d| <= S or |d| >= n - S (|d| < n).
+// holds such a cz exactly when |cz - iz| <= S; across a periodic axis the offset d (|d| < n),
+// folded into (-n/2, n/2], is at most S in magnitude exactly when |d| <= S or |d| >= n - S.
 __device__ __forceinline__ bool near1(int d, int n, int S) {
     const int a = abs(d);
     return a <= S || a >= n - S;
@@ -1087,14 +1096,12 @@ __device__ __forceinline__ Part part(const DevParams& P, float4 x, uint32_t lkey
     return a;
 }
 
-// The support test shared by both orientations (negating r is exact, so both square alike).
-__device__ __forceinline__ bool support(const DevParams& P, float4 xi, float4 xj, float& rx, float& ry, float& rz) {
-    rx = xi.x - xj.x;
-    ry = xi.y - xj.y;
-    rz = xi.z - xj.z;
-    min_image(P, rx, ry);
-    const float dxm = rx * P.inv_dx, dym = ry * P.inv_dy, dzm = rz * P.inv_dz;
-    return __builtin_fmaf(dzm, dzm, __builtin_fmaf(dym, dym, dxm * dxm)) < P.lim;
+// The support test shared by both orientations (negating s is exact, so both square alike), from
+// i's coordinates in cells (ci) and j's position.
+__device__ __forceinline__ bool support(const DevParams& P, float4 ci, float4 xj) {
+    float sx, sy, sz;
+    csep(P, ci, cellpos(P, xj), sx, sy, sz);
+    return r2of(sx, sy, sz) < P.lim;
 }
 
 __device__ __forceinline__ bool rev_role(const DevParams& P, float4 xj, const Part& a) {
@@ -1202,14 +1209,15 @@ constexpr uint16_t QOWN = 0x8000, QREV = 0x4000, QSLOT = 0x3fff;   // LDS queue 
 
 // ---- momentum --------------------------------------------------------------------------------
 struct MomI {
-    float4 xi;
+    float4 xi, ci;   // position, and in cells
     float vi[3], rho_i, rho_raw, p_i, Ai[3], AVi[3];
     float xb[3], vb[3], rb, pb;
 };
 
-__device__ __forceinline__ MomI mom_load(const AdjTape& T, const AdjWork& W, uint32_t i) {
+__device__ __forceinline__ MomI mom_load(const DevParams& P, const AdjTape& T, const AdjWork& W, uint32_t i) {
     MomI m;
     m.xi = T.x1[i];
+    m.ci = cellpos(P, m.xi);
     const float4 v = T.v1[i], A = W.Ab[i], AV = W.AVb[i];
     m.vi[0] = v.x; m.vi[1] = v.y; m.vi[2] = v.z;
     m.rho_raw = T.r1[i];
@@ -1239,14 +1247,14 @@ __device__ __forceinline__ MomJ mom_operands(const AdjTape& T, const AdjWork& W,
 }
 
 __device__ __forceinline__ void mom_partner(const DevParams& P, MomI& m, float4 xj, const MomJ& o, bool own) {
-    float rx = m.xi.x - xj.x, ry = m.xi.y - xj.y, rz = m.xi.z - xj.z;
-    min_image(P, rx, ry);
+    float sx, sy, sz;
+    csep(P, m.ci, cellpos(P, xj), sx, sy, sz);
     const float4 vj4 = o.v;
     const float vj[3] = {vj4.x, vj4.y, vj4.z};
     const float rho_j = fmaxf(o.rho, 1e-3f);
     const float Aj[3] = {o.A.x, o.A.y, o.A.z}, AVj[3] = {o.AV.x, o.AV.y, o.AV.z};
     const float z3[3] = {0.f, 0.f, 0.f};
-    adj::mom_pair_vjp2<float>(P, rx, ry, rz, m.vi, vj, m.rho_i, rho_j, m.p_i, vj4.w, own ? m.Ai : z3,
+    adj::mom_pair_vjp2<float>(P, sx, sy, sz, m.vi, vj, m.rho_i, rho_j, m.p_i, vj4.w, own ? m.Ai : z3,
                               own ? m.AVi : z3, Aj, AVj, m.xb, m.vb, m.rb, m.pb);
 }
 
@@ -1260,12 +1268,11 @@ __device__ __forceinline__ void mom_store(const AdjWork& W, uint32_t i, const Mo
 
 __device__ __forceinline__ void mom_global(const DevParams& P, uint32_t k0, const AdjTape& T, const AdjWork& W,
                                            const float* dl, uint32_t i) {
-    MomI m = mom_load(T, W, i);
+    MomI m = mom_load(P, T, W, i);
     const Part a = part(P, m.xi, T.keys[i] - k0);
     walk_global(P, k0, T.cstart, dl, a, [&](uint32_t j, bool own) {
         const float4 xj = T.x1[j];
-        float rx, ry, rz;
-        if (!support(P, m.xi, xj, rx, ry, rz)) return;
+        if (!support(P, m.ci, xj)) return;
         const bool rev = rev_role(P, xj, a);
         if (own || rev) mom_partner(P, m, xj, mom_operands(T, W, j, rev), own);
     });
@@ -1274,15 +1281,16 @@ __device__ __forceinline__ void mom_global(const DevParams& P, uint32_t k0, cons
 
 // ---- density ---------------------------------------------------------------------------------
 struct DensI {
-    float4 xi;
+    float4 xi, ci;   // position, and in cells
     float r0_raw, inv_r0i;
     float2 sbi;
     float xb[3], rb;
 };
 
-__device__ __forceinline__ DensI dens_load(const AdjTape& T, const AdjWork& W, uint32_t i) {
+__device__ __forceinline__ DensI dens_load(const DevParams& P, const AdjTape& T, const AdjWork& W, uint32_t i) {
     DensI d;
     d.xi = T.x1[i];
+    d.ci = cellpos(P, d.xi);
     d.r0_raw = T.r0[i];
     d.inv_r0i = adj::Fn<float>::rcp(fmaxf(d.r0_raw, 1e-3f));
     d.sbi = W.sb[i];
@@ -1306,10 +1314,10 @@ __device__ __forceinline__ DensJ dens_operands(const AdjTape& T, const AdjWork& 
 
 __device__ __forceinline__ void dens_partner(const DevParams& P, DensI& d, float4 xj, const DensJ& o, bool own,
                                              bool rev) {
-    float rx = d.xi.x - xj.x, ry = d.xi.y - xj.y, rz = d.xi.z - xj.z;
-    min_image(P, rx, ry);
+    float sx, sy, sz;
+    csep(P, d.ci, cellpos(P, xj), sx, sy, sz);
     float Wv, gw[3];
-    if (!adj::dens_pair<float>(P, rx, ry, rz, Wv, gw)) return;
+    if (!adj::dens_pair<float>(P, sx, sy, sz, Wv, gw)) return;
     float wb = 0.f;
     if (own) wb = d.sbi.x + d.sbi.y * adj::Fn<float>::rcp(fmaxf(o.r0, 1e-3f));
     if (rev) {   // (j, i): r_ji = -r_ij, the gradient flips sign and x_i enters with a minus
@@ -1328,12 +1336,11 @@ __device__ __forceinline__ void dens_store(const AdjWork& W, const AdjState& Sst
 
 __device__ __forceinline__ void dens_global(const DevParams& P, uint32_t k0, const AdjTape& T, const AdjWork& W,
                                             const AdjState& Sst, const float* dl, uint32_t i) {
-    DensI d = dens_load(T, W, i);
+    DensI d = dens_load(P, T, W, i);
     const Part a = part(P, d.xi, T.keys[i] - k0);
     walk_global(P, k0, T.cstart, dl, a, [&](uint32_t j, bool own) {
         const float4 xj = T.x1[j];
-        float rx, ry, rz;
-        if (!support(P, d.xi, xj, rx, ry, rz)) return;
+        if (!support(P, d.ci, xj)) return;
         const bool rev = rev_role(P, xj, a);
         if (own || rev) dens_partner(P, d, xj, dens_operands(T, W, j, own, rev), own, rev);
     });
@@ -1391,9 +1398,10 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const uint32_t il = active ? i : o0;
         MomI m;
         DensI d;
-        if (KIND == 0) m = mom_load(T, W, il);
-        else d = dens_load(T, W, il);
+        if (KIND == 0) m = mom_load(P, T, W, il);
+        else d = dens_load(P, T, W, il);
         const float4 xi = KIND == 0 ? m.xi : d.xi;
+        const float4 ci = KIND == 0 ? m.ci : d.ci;
         const Part a = part(P, xi, T.keys[il] - k0);
         int qn = 0;
         // Two queue entries per iteration: their staged positions (and, for density, the
@@ -1423,8 +1431,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (active) {
                 walk_halo<S>(P, g, sm, s, a, [&](int slot, bool own) {
                     const float4 pj = sm.p[slot];
-                    float rx, ry, rz;
-                    if (support(P, xi, pj, rx, ry, rz)) {
+                    if (support(P, ci, pj)) {
                         const bool rev = rev_role(P, pj, a);
                         if (own || rev)
                             sm.q[qn++][tid] = (uint16_t)((uint32_t)slot | (own ? QOWN : 0u) | (rev ? QREV : 0u));
