@@ -540,6 +540,15 @@ __device__ __forceinline__ void plan(const DevParams& P, uint32_t k0, const Bric
         run += v;
     }
     if (tid == 0) sm.off[HC] = (uint16_t)(fits ? total : 0u);
+    if (fits && total <= (uint32_t)HaloMax<S, MOM>::MAXP) {   // workgroup-uniform: one slab
+        if (tid == 0) {
+            sm.slab[0] = 0;
+            sm.slab[1] = (uint8_t)g.Hz;
+            sm.nslab = 1;
+        }
+        __syncthreads();
+        return;
+    }
     __syncthreads();
     if (tid == 0) {
         const int L2 = g.Hx * g.Hy;
