@@ -617,6 +617,12 @@ __device__ __forceinline__ Cur cur_cell(const DevParams& P, const BrickGeo& g, f
 
 // Walks the culled stencil rows of a particle that lie in slab s: calls cand(slot) for every
 // candidate in the reference order. Returns the number of candidates.
+// ez, ey are the distances from the particle to a layer's or row's cells widened by the drift
+// bound, [j - d, j + 1 + d]; rows beyond the support are skipped and the cells of a row are
+// clipped to its chord. These are culling bounds only: their rounding (a few ulps of coordinates
+// below 2^16 cells) and the 1-ulp hardware square root are covered by the 1e-3-cell margin, and
+// every surviving candidate takes the exact support test. All lanes step through the same stencil
+// offsets together (rows of equal offset hold similar spans), skipping rows individually.
 template <int S, class Cand>
 __device__ __forceinline__ int walk_rows(const DevParams& P, const BrickGeo& g, const float* delta,
                                          const uint16_t* off, const uint8_t* slab, int s, const Cur& c,
@@ -627,21 +633,27 @@ __device__ __forceinline__ int walk_rows(const DevParams& P, const BrickGeo& g, 
     const int za = max(c.z0, g.hz0 + (int)slab[s]);
     const int zb = min(c.z1, g.hz0 + (int)slab[s + 1] - 1);
     const int ya = max(c.iy - S, g.hy0), yb = min(c.iy + S, g.hy0 + g.Hy - 1);
+    const int xa = max(c.ix - S, g.hx0), xb = min(c.ix + S, g.hx0 + g.Hx - 1);
+    // a cell j's widened box lies beyond the particle by j - zP on one side, zM - j on the other
+    const float zP = c.zi + delta[2], zM = (c.zi - delta[2]) - 1.0f;
+    const float yP = c.yi + delta[1], yM = (c.yi - delta[1]) - 1.0f;
+    const float xP = c.xi + delta[0], xM = (c.xi - delta[0]) - 1.0f;
+    const float fya = (float)ya;
     int ncand = 0;
     for (int jz = za; jz <= zb; ++jz) {
-        float ez = fmaxf(0.f, fmaxf(((float)jz - delta[2]) - c.zi, c.zi - ((float)(jz + 1) + delta[2])));
-        float ez2 = ez * ez;
-        const int rowz = (jz - g.hz0) * g.Hy;
-        for (int jy = ya; jy <= yb; ++jy) {
-            float ey = fmaxf(0.f, fmaxf(((float)jy - delta[1]) - c.yi, c.yi - ((float)(jy + 1) + delta[1])));
-            float d2 = __builtin_fmaf(ey, ey, ez2);
+        const float fz = (float)jz;
+        const float ez = fmaxf(0.f, fmaxf(fz - zP, zM - fz));
+        const float ez2 = ez * ez;
+        if (ez2 >= limc) continue;
+        int row = ((jz - g.hz0) * g.Hy + (ya - g.hy0)) * g.Hx - g.hx0;
+        float fy = fya;
+        for (int jy = ya; jy <= yb; ++jy, fy += 1.0f, row += g.Hx) {
+            const float ey = fmaxf(0.f, fmaxf(fy - yP, yM - fy));
+            const float d2 = __builtin_fmaf(ey, ey, ez2);
             if (d2 >= limc) continue;
-            // a culling bound only (the 1e-3-cell margin covers the 1-ulp hardware square root)
-            float ext = __builtin_amdgcn_sqrtf(limc - d2) + DELTA_EPS;
-            int xlo = max(max(c.ix - S, g.hx0), (int)ceilf(c.xi - ext - 1.0f - delta[0]));
-            int xhi = min(min(c.ix + S, g.hx0 + g.Hx - 1), (int)floorf(c.xi + ext + delta[0]));
+            const float ext = __builtin_amdgcn_sqrtf(limc - d2) + DELTA_EPS;
+            const int xlo = max(xa, (int)ceilf(xM - ext)), xhi = min(xb, (int)floorf(xP + ext));
             if (xlo > xhi) continue;
-            const int row = (rowz + (jy - g.hy0)) * g.Hx - g.hx0;
             const int a = off[row + xlo] - base, e = off[row + xhi + 1] - base;
             ncand += e - a;
             // two candidates per step (the second absent at the end of an odd span)
