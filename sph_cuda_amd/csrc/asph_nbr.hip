@@ -1203,11 +1203,16 @@ __device__ __forceinline__ void walk_halo(const DevParams& P, const BrickGeo& g,
             if (xl > xh) continue;
             const bool oyz = oz && abs(cyu - iyu) <= S;
             const int row = (z * g.Hy + y) * g.Hx;
-            for (int x = xl; x <= xh; ++x) {
-                const bool own = oyz && x >= oxl && x <= oxh;
-                const int p0 = (int)sm.off[row + x] - base, p1 = (int)sm.off[row + x + 1] - base;
-                for (int p = p0; p < p1; ++p) f(p, own);
+            // the row's cells xl..xh are one span of slots; its own cells (x in [oxl, oxh], when
+            // the row is in the stencil of i's current cell) one contiguous part of it
+            const int p0 = (int)sm.off[row + xl] - base, p1 = (int)sm.off[row + xh + 1] - base;
+            const int ol = max(xl, oxl), oh = min(xh, oxh);
+            int q0 = p1, q1 = p1;
+            if (oyz && ol <= oh) {
+                q0 = (int)sm.off[row + ol] - base;
+                q1 = (int)sm.off[row + oh + 1] - base;
             }
+            for (int p = p0; p < p1; ++p) f(p, p >= q0 && p < q1);
         }
     }
 }
