@@ -387,8 +387,8 @@ template <int S, bool MOM> struct HaloMax {
 
 template <int S, bool MOM> struct Smem {
     float4 p[HaloMax<S, MOM>::MAXP];   // x/dx, y/dy, z/dz (cells), and 1/max(rho_old,1e-3)
-                                       // (density) or the particle's global index (momentum);
-                                       // the adjoint stages x, y, z unscaled
+                                       // (density) or the particle's global index (momentum
+                                       // and the reverse pair stages)
     uint16_t off[HaloMax<S, MOM>::HC + 1];   // exclusive start of each halo cell over the halo
     uint16_t q[queue_len<MOM>()][BT];  // per-thread queue of LDS slots
     uint32_t mk[3][HaloMax<S, MOM>::H];  // Morton bits of the halo's x, y, z cells (wrapped);
@@ -1118,16 +1118,19 @@ __device__ __forceinline__ Part part(const DevParams& P, float4 x, uint32_t lkey
 }
 
 // The support test shared by both orientations (negating s is exact, so both square alike), from
-// i's coordinates in cells (ci) and j's position.
-__device__ __forceinline__ bool support(const DevParams& P, float4 ci, float4 xj) {
+// the coordinates in cells of i (ci) and j (cj).
+__device__ __forceinline__ bool support(const DevParams& P, float4 ci, float4 cj) {
     float sx, sy, sz;
-    csep(P, ci, cellpos(P, xj), sx, sy, sz);
+    csep(P, ci, cj, sx, sy, sz);
     return r2of(sx, sy, sz) < P.lim;
 }
 
-__device__ __forceinline__ bool rev_role(const DevParams& P, float4 xj, const Part& a) {
-    int jx, jy, jz;
-    cell_trunc(P, xj.x, xj.y, xj.z, jx, jy, jz);
+// Whether i is listed in the stencil of j's current cell. The cell is cell_trunc's, taken from j's
+// coordinates in cells (the same products x * inv_dx).
+__device__ __forceinline__ bool rev_role(const DevParams& P, float4 cj, const Part& a) {
+    const int jx = min(max((int)cj.x, 0), P.nx - 1);
+    const int jy = min(max((int)cj.y, 0), P.ny - 1);
+    const int jz = min(max((int)cj.z, 0), P.nz - 1);
     return in_stencil(P, jx, jy, jz, a.lx, a.ly, a.lz);
 }
 
@@ -1272,9 +1275,9 @@ __device__ __forceinline__ MomJ mom_operands(const AdjTape& T, const AdjWork& W,
     return o;
 }
 
-__device__ __forceinline__ void mom_partner(const DevParams& P, MomI& m, float4 xj, const MomJ& o, bool own) {
+__device__ __forceinline__ void mom_partner(const DevParams& P, MomI& m, float4 cj, const MomJ& o, bool own) {
     float sx, sy, sz;
-    csep(P, m.ci, cellpos(P, xj), sx, sy, sz);
+    csep(P, m.ci, cj, sx, sy, sz);
     const float4 vj4 = o.v;
     const float vj[3] = {vj4.x, vj4.y, vj4.z};
     const float rho_j = fmaxf(o.rho, 1e-3f);
@@ -1297,10 +1300,10 @@ __device__ __forceinline__ void mom_global(const DevParams& P, uint32_t k0, cons
     MomI m = mom_load(P, T, W, i);
     const Part a = part(P, m.xi, T.keys[i] - k0);
     walk_global(P, k0, T.cstart, dl, a, [&](uint32_t j, bool own) {
-        const float4 xj = T.x1[j];
-        if (!support(P, m.ci, xj)) return;
-        const bool rev = rev_role(P, xj, a);
-        if (own || rev) mom_partner(P, m, xj, mom_operands(T, W, j, rev), own);
+        const float4 cj = cellpos(P, T.x1[j]);
+        if (!support(P, m.ci, cj)) return;
+        const bool rev = rev_role(P, cj, a);
+        if (own || rev) mom_partner(P, m, cj, mom_operands(T, W, j, rev), own);
     });
     mom_store(W, i, m);
 }
@@ -1338,10 +1341,10 @@ __device__ __forceinline__ DensJ dens_operands(const AdjTape& T, const AdjWork& 
     return o;
 }
 
-__device__ __forceinline__ void dens_partner(const DevParams& P, DensI& d, float4 xj, const DensJ& o, bool own,
+__device__ __forceinline__ void dens_partner(const DevParams& P, DensI& d, float4 cj, const DensJ& o, bool own,
                                              bool rev) {
     float sx, sy, sz;
-    csep(P, d.ci, cellpos(P, xj), sx, sy, sz);
+    csep(P, d.ci, cj, sx, sy, sz);
     float Wv, gw[3];
     if (!adj::dens_pair<float>(P, sx, sy, sz, Wv, gw)) return;
     float wb = 0.f;
@@ -1365,10 +1368,10 @@ __device__ __forceinline__ void dens_global(const DevParams& P, uint32_t k0, con
     DensI d = dens_load(P, T, W, i);
     const Part a = part(P, d.xi, T.keys[i] - k0);
     walk_global(P, k0, T.cstart, dl, a, [&](uint32_t j, bool own) {
-        const float4 xj = T.x1[j];
-        if (!support(P, d.ci, xj)) return;
-        const bool rev = rev_role(P, xj, a);
-        if (own || rev) dens_partner(P, d, xj, dens_operands(T, W, j, own, rev), own, rev);
+        const float4 cj = cellpos(P, T.x1[j]);
+        if (!support(P, d.ci, cj)) return;
+        const bool rev = rev_role(P, cj, a);
+        if (own || rev) dens_partner(P, d, cj, dens_operands(T, W, j, own, rev), own, rev);
     });
     dens_store(W, Sst, i, d);
 }
@@ -1414,7 +1417,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         return;
     }
     auto rec = [&](Smem<S, true>& s, uint32_t slot, uint32_t jg, float4 pj) {
-        s.p[slot] = make_float4(pj.x, pj.y, pj.z, __uint_as_float(jg));
+        s.p[slot] = make_float4(pj.x * P.inv_dx, pj.y * P.inv_dy, pj.z * P.inv_dz, __uint_as_float(jg));
     };
     const int ns = sm.nslab;
     if (ns == 1) stage_slab<S, true>(P, k0, g, 0, T.x1, T.cstart, sm, rec);
@@ -1456,9 +1459,9 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             if (ns > 1) stage_slab<S, true>(P, k0, g, s, T.x1, T.cstart, sm, rec);
             if (active) {
                 walk_halo<S>(P, g, sm, s, a, [&](int slot, bool own) {
-                    const float4 pj = sm.p[slot];
-                    if (support(P, ci, pj)) {
-                        const bool rev = rev_role(P, pj, a);
+                    const float4 cj = sm.p[slot];
+                    if (support(P, ci, cj)) {
+                        const bool rev = rev_role(P, cj, a);
                         if (own || rev)
                             sm.q[qn++][tid] = (uint16_t)((uint32_t)slot | (own ? QOWN : 0u) | (rev ? QREV : 0u));
                     }
