@@ -18,11 +18,12 @@
 // the cell table.
 //
 // A halo holding more than MAXP particles (clumped or compressed regions) is streamed through
-// LDS in slabs of whole z-layers: each thread keeps its sums in registers and walks, slab by
-// slab, the rows of its stencil that lie in the staged layers, so the candidate order is still
-// the reference's. Only a brick with a single halo layer beyond MAXP, or more than 65534 halo
-// particles, runs the thread-per-particle loop over the global cell table, as does mode B
-// (drift since the last sort >= 0.95 cell, or a domain narrower than a halo).
+// LDS in slabs of consecutive halo rows (z-major, then y): each thread keeps its sums in
+// registers and walks, slab by slab, the rows of its stencil that lie in the staged slab, so the
+// candidate order is still the reference's. Only a brick with a single halo row beyond MAXP, more
+// than NSLAB slabs, or more than 65534 halo particles, runs the thread-per-particle loop over the
+// global cell table, as does mode B (drift since the last sort >= 0.95 cell, or a domain
+// narrower than a halo).
 //
 // Candidate culling is exact: particles listed in cell c have moved at most delta cells (the
 // device-tracked maximum drift since the sort) so they lie within [c - delta, c + 1 + delta] on
@@ -52,6 +53,7 @@ constexpr int DU = 3;              // queue entries per drain iteration, density
 constexpr int MU = 3;              // and momentum
 static_assert(QKD % DU == 0 && QK % MU == 0 && QK % 2 == 0, "a drain iteration never reads past the queue");
 template <bool MOM> constexpr int queue_len() { return MOM ? QK : QKD; }
+constexpr int NSLAB = 64;          // most slabs of one halo (beyond: the global-memory loop)
 constexpr int CHUNK_MIN = 256;     // bounds of the work-item size (tuned at run time)
 constexpr int CHUNK_MAX = 1024;
 constexpr float DELTA_EPS = 1e-3f; // rounding margin of the culling bounds [cells]
@@ -394,8 +396,10 @@ template <int S, bool MOM> struct Smem {
     uint32_t mk[3][HaloMax<S, MOM>::H];  // Morton bits of the halo's x, y, z cells (wrapped);
                                          // z: ~0u where the layer lies outside [0, nz)
     uint32_t wsum[BW];
-    uint8_t slab[HaloMax<S, MOM>::H + 1];    // slab boundaries in halo z-layers
+    uint16_t slab[NSLAB + 1];          // slab boundaries in halo rows (z * Hy + y)
+    uint8_t slz[2][NSLAB];             // first and last halo z-layer of each slab
     int nslab;                         // 0: the halo cannot be staged (global-memory loop)
+    int rowcut;                        // 1: slabs end inside z-layers (a layer exceeds MAXP)
     int modeA;                         // drift bound below 0.95 cell on every axis
     float delta[3];                    // drift bound since the last sort + rounding margin [cells]
 };
@@ -494,8 +498,9 @@ __device__ __forceinline__ void halo_tables(const DevParams& P, const BrickGeo& 
 }
 
 // Counts the halo's particles per cell, turns the counts into start offsets over the whole
-// halo, and splits the halo into slabs of whole z-layers holding at most MAXP particles each.
-// Sets sm.nslab = 0 if that is impossible (16-bit offsets or one layer beyond MAXP).
+// halo, and splits the halo into slabs of consecutive halo rows (z-major, then y) holding at most
+// MAXP particles each. Sets sm.nslab = 0 if that is impossible (16-bit offsets, one row beyond
+// MAXP, or more than NSLAB slabs).
 template <int S, bool MOM>
 __device__ __forceinline__ void plan(const DevParams& P, uint32_t k0, const BrickGeo& g,
                                      const uint32_t* __restrict__ cstart, Smem<S, MOM>& sm) {
@@ -540,10 +545,14 @@ __device__ __forceinline__ void plan(const DevParams& P, uint32_t k0, const Bric
         run += v;
     }
     if (tid == 0) sm.off[HC] = (uint16_t)(fits ? total : 0u);
+    const int R = g.Hz * g.Hy;   // halo rows
     if (fits && total <= (uint32_t)HaloMax<S, MOM>::MAXP) {   // workgroup-uniform: one slab
         if (tid == 0) {
             sm.slab[0] = 0;
-            sm.slab[1] = (uint8_t)g.Hz;
+            sm.slab[1] = (uint16_t)R;
+            sm.slz[0][0] = 0;
+            sm.slz[1][0] = (uint8_t)(g.Hz - 1);
+            sm.rowcut = 0;
             sm.nslab = 1;
         }
         __syncthreads();
@@ -551,17 +560,39 @@ __device__ __forceinline__ void plan(const DevParams& P, uint32_t k0, const Bric
     }
     __syncthreads();
     if (tid == 0) {
-        const int L2 = g.Hx * g.Hy;
-        int ns = 0, z = 0;
+        // greedy: each slab takes the most whole z-layers that fit (rowcut = 0), or where one
+        // layer alone exceeds MAXP the most rows (rowcut = 1); a slab's end is found by bisection
+        // over the layer or row starts (off[r * Hx] is nondecreasing in r)
+        const uint32_t cap = (uint32_t)HaloMax<S, MOM>::MAXP;
+        int ns = 0;
         bool ok = fits;
-        sm.slab[0] = 0;
-        while (ok && z < g.Hz) {
-            const uint32_t b = sm.off[z * L2];
-            int z1 = z;
-            while (z1 < g.Hz && (uint32_t)sm.off[(z1 + 1) * L2] - b <= (uint32_t)HaloMax<S, MOM>::MAXP) ++z1;
-            if (z1 == z) ok = false;
-            else { sm.slab[++ns] = (uint8_t)z1; z = z1; }
+        int rowcut = 0;
+        for (int pass = 0; pass < 2 && ok; ++pass) {
+            const int unit = pass == 0 ? g.Hy : 1;   // rows per packing unit
+            const int U = R / unit;
+            ns = 0;
+            int r = 0;
+            sm.slab[0] = 0;
+            bool done = true;
+            while (r < U) {
+                const uint32_t b = sm.off[r * unit * g.Hx];
+                int lo = r, hi = U;   // the last unit end with at most MAXP particles after b
+                while (lo < hi) {
+                    const int mid = (lo + hi + 1) >> 1;
+                    if ((uint32_t)sm.off[mid * unit * g.Hx] - b <= cap) lo = mid;
+                    else hi = mid - 1;
+                }
+                if (lo == r || ns == NSLAB) { done = false; break; }
+                sm.slz[0][ns] = (uint8_t)(r * unit / g.Hy);
+                sm.slz[1][ns] = (uint8_t)((lo * unit - 1) / g.Hy);
+                sm.slab[++ns] = (uint16_t)(lo * unit);
+                r = lo;
+            }
+            if (done) break;
+            if (pass == 1 || ns == NSLAB) ok = false;   // one row beyond MAXP, or too many slabs
+            rowcut = 1;
         }
+        sm.rowcut = rowcut;
         sm.nslab = ok ? ns : 0;
     }
     __syncthreads();
@@ -574,8 +605,7 @@ __device__ __forceinline__ void stage_slab(const DevParams& P, uint32_t k0, cons
                                            const float4* __restrict__ pos, const uint32_t* __restrict__ cstart,
                                            Smem<S, MOM>& sm, Rec rec) {
     __syncthreads();
-    const int L2 = g.Hx * g.Hy;
-    const int cb = sm.slab[s] * L2, ce = sm.slab[s + 1] * L2;
+    const int cb = sm.slab[s] * g.Hx, ce = sm.slab[s + 1] * g.Hx;
     const uint32_t base = sm.off[cb];
     for (int c = cb + (int)threadIdx.x; c < ce; c += BT) {
         const uint32_t o = sm.off[c], n = (uint32_t)sm.off[c + 1] - o;
@@ -622,16 +652,20 @@ __device__ __forceinline__ Cur cur_cell(const DevParams& P, const BrickGeo& g, f
 // clipped to its chord. These are culling bounds only: their rounding (a few ulps of coordinates
 // below 2^16 cells) and the 1-ulp hardware square root are covered by the 1e-3-cell margin, and
 // every surviving candidate takes the exact support test. All lanes step through the same stencil
-// offsets together (rows of equal offset hold similar spans), skipping rows individually.
-template <int S, class Cand>
+// offsets together (rows of equal offset hold similar spans), skipping rows individually; rows
+// outside slab s are skipped the same way when the slab is part of the halo.
+template <int S, bool ROWS, class Cand, class Sm>
 __device__ __forceinline__ int walk_rows(const DevParams& P, const BrickGeo& g, const float* delta,
-                                         const uint16_t* off, const uint8_t* slab, int s, const Cur& c,
-                                         Cand cand) {
+                                         const Sm& sm, int s, const Cur& c, Cand cand) {
     const float limc = P.lim * 1.0001f;
-    const int L2 = g.Hx * g.Hy;
-    const int base = off[slab[s] * L2];
-    const int za = max(c.z0, g.hz0 + (int)slab[s]);
-    const int zb = min(c.z1, g.hz0 + (int)slab[s + 1] - 1);
+    const uint16_t* off = sm.off;
+    // the slab's halo rows [r0, r1) hold its slots [0, cnt). With ROWS (slabs that end inside
+    // z-layers) the rows of its first and last layers outside it clamp to an empty span.
+    const int r0 = sm.slab[s], r1 = sm.slab[s + 1];
+    const int base = __builtin_amdgcn_readfirstlane((int)off[r0 * g.Hx]);
+    const int cnt = __builtin_amdgcn_readfirstlane((int)off[r1 * g.Hx]) - base;
+    const int za = max(c.z0, g.hz0 + (int)sm.slz[0][s]);
+    const int zb = min(c.z1, g.hz0 + (int)sm.slz[1][s]);
     const int ya = max(c.iy - S, g.hy0), yb = min(c.iy + S, g.hy0 + g.Hy - 1);
     const int xa = max(c.ix - S, g.hx0), xb = min(c.ix + S, g.hx0 + g.Hx - 1);
     // a cell j's widened box lies beyond the particle by j - zP on one side, zM - j on the other
@@ -654,7 +688,11 @@ __device__ __forceinline__ int walk_rows(const DevParams& P, const BrickGeo& g, 
             const float ext = __builtin_amdgcn_sqrtf(limc - d2) + DELTA_EPS;
             const int xlo = max(xa, (int)ceilf(xM - ext)), xhi = min(xb, (int)floorf(xP + ext));
             if (xlo > xhi) continue;
-            const int a = off[row + xlo] - base, e = off[row + xhi + 1] - base;
+            int a = (int)off[row + xlo] - base, e = (int)off[row + xhi + 1] - base;
+            if (ROWS) {
+                a = min(max(a, 0), cnt);
+                e = min(max(e, 0), cnt);
+            }
             ncand += e - a;
             // two candidates per step (the second absent at the end of an odd span)
             for (int p = a; p < e; p += 2) cand(p, p + 1 < e);
@@ -685,7 +723,7 @@ __device__ __forceinline__ void count_wg(unsigned long long* stats, int idx, int
 // ---------------------------------------------------------------------------------------------
 // Density + EOS + microphysics.
 // ---------------------------------------------------------------------------------------------
-template <int S, bool WRAP, class Rec>
+template <int S, bool WRAP, bool ROWS, class Rec>
 __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A, const DensConst& K,
                                            uint32_t k0, const BrickGeo& g, uint32_t o0, uint32_t o1,
                                            Smem<S, false>& sm, Rec rec) {
@@ -737,7 +775,7 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
             if (active) {
                 // candidates p and p + 1 (if two): both positions are read before either test;
                 // the queue takes them in order and is drained before it could overflow
-                ncand += walk_rows<S>(P, g, sm.delta, sm.off, sm.slab, s, cc, [&](int p, bool two) {
+                ncand += walk_rows<S, ROWS>(P, g, sm.delta, sm, s, cc, [&](int p, bool two) {
                     const float4 pj0 = sm.p[p], pj1 = sm.p[two ? p + 1 : p];
                     float dxm, dym, dzm;
                     diff(pj0, dxm, dym, dzm);
@@ -798,10 +836,11 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
 // Workgroup b plays up to two roles: mode B for brick b (members not in mode A), and mode A for
 // work item b (members in mode A). Every particle is computed by exactly one of them.
 template <int S>
-__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_density_eos_brick(
+__global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6))) void k_density_eos_brick(
         const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
         const uint32_t* __restrict__ nitems, int nbblk, int n_members, int csize, DensArgs A0) {
     __shared__ Smem<S, false> sm;
+    static_assert(sizeof(Smem<S, false>) <= HaloMax<S, false>::LDS, "density LDS budget");
     DensArgs A = A0;
     if (A.ctl) {   // device controller: its dt; an idle step only rotates the density buffers
         if (A.ctl->idle) {
@@ -870,14 +909,19 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     auto rec = [&](Smem<S, false>& s, uint32_t slot, uint32_t jg, float4 pj) {
         s.p[slot] = make_float4(pj.x * P.inv_dx, pj.y * P.inv_dy, pj.z * P.inv_dz, 1.0f / fmaxf(rho_old[jg], 1e-3f));
     };
-    if (g.wraps) dens_brick<S, true>(P, A, K, k0, g, o0, o1, sm, rec);
-    else dens_brick<S, false>(P, A, K, k0, g, o0, o1, sm, rec);
+    if (sm.rowcut) {
+        if (g.wraps) dens_brick<S, true, true>(P, A, K, k0, g, o0, o1, sm, rec);
+        else dens_brick<S, false, true>(P, A, K, k0, g, o0, o1, sm, rec);
+    } else {
+        if (g.wraps) dens_brick<S, true, false>(P, A, K, k0, g, o0, o1, sm, rec);
+        else dens_brick<S, false, false>(P, A, K, k0, g, o0, o1, sm, rec);
+    }
 }
 
 // ---------------------------------------------------------------------------------------------
 // Momentum + theta update + kick + sponge + max|v|.
 // ---------------------------------------------------------------------------------------------
-template <int S, bool WRAP, class Rec>
+template <int S, bool WRAP, bool ROWS, class Rec>
 __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, uint32_t k0, const BrickGeo& g,
                                           uint32_t o0, uint32_t o1, Smem<S, true>& sm, Rec rec,
                                           float& mx, float& my, float& mz) {
@@ -944,7 +988,7 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
             if (ns > 1) stage_slab<S, true>(P, k0, g, s, A.pos, A.cstart, sm, rec);
             if (active) {
                 // candidates p and p + 1 (if two), as in the density kernel
-                ncand += walk_rows<S>(P, g, sm.delta, sm.off, sm.slab, s, cc, [&](int p, bool two) {
+                ncand += walk_rows<S, ROWS>(P, g, sm.delta, sm, s, cc, [&](int p, bool two) {
                     const float4 pj0 = sm.p[p], pj1 = sm.p[two ? p + 1 : p];
                     float dxt, dyt, dzt;
                     diff(pj0, dxt, dyt, dzt);
@@ -989,6 +1033,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
         const uint32_t* __restrict__ nitems, int nbblk, int csize, MomArgs A0) {
     __shared__ Smem<S, true> sm;
+    static_assert(sizeof(Smem<S, true>) <= HaloMax<S, true>::LDS, "momentum LDS budget");
     MomArgs A = A0;
     if (A.ctl) {   // device controller: its dt; an idle step only rotates the velocity buffers
         if (A.ctl->idle) {
@@ -1051,8 +1096,13 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
         auto rec = [&](Smem<S, true>& s, uint32_t slot, uint32_t jg, float4 pj) {
             s.p[slot] = make_float4(pj.x * P.inv_dx, pj.y * P.inv_dy, pj.z * P.inv_dz, __uint_as_float(jg));
         };
-        if (g.wraps) mom_brick<S, true>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
-        else mom_brick<S, false>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
+        if (sm.rowcut) {
+            if (g.wraps) mom_brick<S, true, true>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
+            else mom_brick<S, false, true>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
+        } else {
+            if (g.wraps) mom_brick<S, true, false>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
+            else mom_brick<S, false, false>(P, A, k0, g, o0, o1, sm, rec, mx, my, mz);
+        }
     }
     if (A.mode) reduce_maxv(A.maxv + bm.m * 192, mx, my, mz);
 }
@@ -1179,9 +1229,9 @@ __device__ __forceinline__ void walk_halo(const DevParams& P, const BrickGeo& g,
     const float limc = P.lim * 1.0001f;
     const int mx = (g.Hx - BS) / 2, my = (g.Hy - BS) / 2, mz = (g.Hz - BS) / 2;
     const int hx = a.lx - g.hx0, hy = a.ly - g.hy0, hz = a.lz - g.hz0;
-    const int L2 = g.Hx * g.Hy;
-    const int base = sm.off[sm.slab[s] * L2];
-    const int za = max(hz - mz, (int)sm.slab[s]), zb = min(hz + mz, (int)sm.slab[s + 1] - 1);
+    const int r0 = sm.slab[s], r1 = sm.slab[s + 1];   // the slab's halo rows [r0, r1)
+    const int base = sm.off[r0 * g.Hx];
+    const int za = max(hz - mz, r0 / g.Hy), zb = min(hz + mz, (r1 - 1) / g.Hy);
     const int sx = a.ix - a.lx > P.nx / 2 ? -P.nx : (a.lx - a.ix > P.nx / 2 ? P.nx : 0);
     const int sy = a.iy - a.ly > P.ny / 2 ? -P.ny : (a.ly - a.iy > P.ny / 2 ? P.ny : 0);
     const float xu = a.xc + (float)sx, yu = a.yc + (float)sy;
@@ -1195,6 +1245,8 @@ __device__ __forceinline__ void walk_halo(const DevParams& P, const BrickGeo& g,
         if (ez2 >= limc) continue;
         const bool oz = abs(cz - a.iz) <= S;
         for (int y = hy - my; y <= hy + my; ++y) {
+            const int hr = z * g.Hy + y;
+            if (hr < r0 || hr >= r1) continue;
             const int cyu = g.hy0 + y;
             const float ey = box_gap(yu - ((float)cyu + 0.5f), sm.delta[1], 0);
             const float eyz = __builtin_fmaf(ey, ey, ez2);

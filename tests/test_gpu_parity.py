@@ -192,7 +192,7 @@ def test_cfl_trip_and_abort(strict):
 
 def test_clumped_overflow_fallback():
     """A third of the particles collapsed onto other cells: bricks whose halo exceeds the LDS
-    capacity take the global-memory fallback; results are unchanged."""
+    capacity stream it in slabs of halo rows; results are unchanged."""
     cfg, ic, dt = small_config("moist", (24, 24, 16))
     st = s.ic_generate(cfg, ic)
     rng = np.random.default_rng(11)
@@ -227,9 +227,11 @@ def _compressed(a):
 
 
 # (compression, the path the densest bricks must take): a halo beyond the LDS capacity is
-# streamed through it in z-slabs; a brick with one halo layer beyond it (here ~39 particles per
-# cell) takes the global-memory loop
-SPLITS = [((1.2, 1.2, 1.2), "slabbed"), ((1.7, 1.7, 1.7), "slabbed"), ((2.2, 2.2, 8.0), "global")]
+# streamed through it in slabs of halo rows (at 2.2 x 2.2 x 8, ~39 particles per cell, a halo
+# layer alone exceeds it); a brick with one halo row beyond it (at 16 x 16 x 12 all particles in
+# 2 x 2 x 2 cells, 3072 per cell) takes the global-memory loop
+SPLITS = [((1.2, 1.2, 1.2), "slabbed"), ((1.7, 1.7, 1.7), "slabbed"), ((2.2, 2.2, 8.0), "slabbed"),
+          ((16.0, 16.0, 12.0), "global")]
 
 
 @pytest.mark.parametrize("a,path", SPLITS)
