@@ -79,6 +79,38 @@ def test_matches_host_adjoint_on_fp32_trajectory(case):
     assert abs(r["dJ_dqc0"] - gq32) <= 2e-5 * abs(gq32)
 
 
+# (compression of the 20 x 20 x 16 adjoint case towards the bubble centre, which keeps the cloud
+# at its height): the reverse pair stages' halo exceeds the LDS capacity and is streamed in slabs
+# of whole z-layers (x3 in z: ~1300 particles per layer), or of halo rows where one layer alone
+# exceeds it (x16 in z: all 6400 particles in one layer)
+DENSE = [(1.0, 1.0, 3.0), (1.0, 1.0, 16.0)]
+
+
+@pytest.mark.parametrize("a", DENSE)
+def test_dense_halo_slabs(a):
+    """Compressed states whose reverse pair stages stream their halos through LDS in slabs: the
+    device against the host adjoint along the FP32 trajectory (the same forward bit for bit)."""
+    cfg, ic, dt = small_config("adj", SHAPE)
+    st = s.ic_generate(cfg, ic)
+    c = np.array([ic.bubble_x, ic.bubble_y, ic.bubble_z], np.float32)
+    st.pos[:, :3] = (c + (st.pos[:, :3] - c) / np.asarray(a, np.float32)).astype(np.float32)
+    nsteps, ck = 4, 2
+    print(f"[case] adjoint {SHAPE} compressed {a} steps {nsteps} checkpoint {ck}", flush=True)
+    with s.Sim(cfg) as sim:
+        sim.set_state(st)
+        r = sim.adjoint_rain(nsteps, dt, ck)
+    J32, g32, gq32 = adjoint_rain_f64(cfg, st, nsteps, dt, ck, trajectory="f32",
+                                      float_stages=DEVICE_FLOAT_STAGES)
+    g = r["grad_thq"].astype(np.float64)
+    rel = [_normwise(g[:, k], g32[:, k]) for k in range(3)]
+    print(f"[case] normwise theta/qv/qc {rel}, J {r['J']:.12e} vs {J32:.12e}", flush=True)
+    assert J32 > 0
+    assert r["J"] == J32
+    assert np.all(np.isfinite(g))
+    assert max(rel) <= 2e-5
+    assert abs(r["dJ_dqc0"] - gq32) <= 2e-5 * abs(gq32)
+
+
 def test_matches_central_differences(case):
     cfg, dt, st, r, _ = case
     g = r["grad_theta"].astype(np.float64)
