@@ -120,9 +120,10 @@ def cpu_baselines(c1_steps=120, c2_steps=10, adj_steps=10):
     return out
 
 
-def controlled_run(sim, sim_end, report_every=0, wall_cap=0.0):
+def controlled_run(sim, sim_end, report_every=0, wall_cap=0.0, counters=False):
     """The reference loop under its CFL controller from the handle's state to sim_end [s];
-    with report_every, records the run every that many steps (stopping after wall_cap s)."""
+    with report_every, records the run every that many steps (stopping after wall_cap s), with
+    the neighbour kernels' work counters when counters is set."""
     rows = []
     t0 = time.perf_counter()
     last = [t0, 0]
@@ -135,6 +136,11 @@ def controlled_run(sim, sim_end, report_every=0, wall_cap=0.0):
                          ms_per_step=round((now - last[0]) / max(n, 1) * 1e3, 3), sorts=int(ev.sorts),
                          trips=int(ev.trips), umax=[round(float(x), 2) for x in ev.umax],
                          kernel_ms_per_launch={k: round(v[0] / max(v[1], 1), 3) for k, v in kt.items()}))
+        if counters:
+            # work items of the momentum launches by path since the last record
+            c = sim.counters()
+            rows[-1]["mom_items"] = {"lds": c["mom_one_slab"] + c["mom_slabbed"], "global": c["mom_global"],
+                                     "mode_b": c["mom_modeb"]}
         log(f"[bench] {json.dumps(rows[-1])}")
         last[0], last[1] = time.perf_counter(), ev.step
         return bool(wall_cap) and (now - t0) > wall_cap
@@ -552,7 +558,10 @@ def main():
         tune = sim.autotune(reps=2)
         sim.set_profiling(True)
         sim.kernel_times()
-        rc, ts, wall, rows = controlled_run(sim, args.timeline, args.timeline_every, args.timeline_wall)
+        sim.set_counters(True)
+        sim.counters()
+        rc, ts, wall, rows = controlled_run(sim, args.timeline, args.timeline_every, args.timeline_wall,
+                                            counters=True)
         print(json.dumps({"timeline": rows, "rc": rc, "steps": int(ts.steps), "simt_end_s": ts.simt,
                           "wall_s": wall, "value": cfg.n * ts.steps / wall, "tuning": tune["best"],
                           "stopped_by_wall_cap": ts.simt < args.timeline - 1e-6}), flush=True)

@@ -195,6 +195,9 @@ ASPH_DH int64_t to_fx(float v) {
 // Host: fill DevParams from a configuration (float evaluation, the order matches the oracle).
 void make_params(const asph_config& c, DevParams& P);
 
+// Morton bits of one brick of the neighbour kernels: 8 x 8 x 8 cells, keys [base, base + 512).
+constexpr int BRICK_BITS = 9;
+
 // One domain of a handle. A plain run is a handle with one member; an ensemble concatenates its
 // members' particles (and cells, one per particle) and columns, and prefixes their Morton keys
 // with the member index so that one sort keeps every member contiguous and cell-ordered.

@@ -44,6 +44,9 @@ struct Launch {
     uint32_t* ngaps;         // reset by the keys kernel of each sort
     uint32_t gapcap;
     const StepCtl* ctl = nullptr;   // device controller of the launches (asph_run_device), or null
+    // drift bound per brick: how far the particles listed in a brick lie beyond their listed cells
+    float4* wexc = nullptr;  // [npblk * particle_block_size() / 64]: per-wavefront maxima (kick/drift)
+    float4* bexc = nullptr;  // [nkeys >> brick_bits()]: per-brick maxima, indexed by key >> brick_bits()
 };
 
 struct DensArgs {
@@ -60,6 +63,7 @@ struct DensArgs {
     uint32_t* maxv;              // [n_members][3][64], cleared here for the momentum kernel
     const float* disp;
     const uint32_t* disp_slots;
+    const float4* bexc = nullptr;   // per-brick drift bound (Launch::bexc), or null: the member-wide one
     const double* clock;         // simulated time at the start of the step (device), or null:
     double simt;                 // then this value is used
     unsigned long long* stats;   // work counters (ASPH_C_DENS_*), or null
@@ -84,6 +88,7 @@ struct MomArgs {
     uint32_t* maxv;              // [n_members][3][64]
     const float* disp;
     const uint32_t* disp_slots;
+    const float4* bexc = nullptr;   // per-brick drift bound (Launch::bexc), or null: the member-wide one
     unsigned long long* stats;   // work counters (ASPH_C_MOM_*), or null
     float4* av_out;              // adjoint tape: the viscous acceleration sums (x mm), or null
     const StepCtl* ctl = nullptr;   // device controller (dt, idle), or null
@@ -152,8 +157,12 @@ void launch_step_begin(hipStream_t s, float* disp, uint32_t* disp_slots, int res
 void launch_ctl_pre(hipStream_t s, StepCtl* ctl);
 void launch_ctl_post(hipStream_t s, StepCtl* ctl, const uint32_t* maxv);
 void launch_clamp_vel_ctl(const Launch& L, hipStream_t s, float4* vel);
+// Kick and drift. With the sorted keys and the cell table of the current sort (keys non-null), it
+// also records, per brick, how far the particles listed in it now lie beyond their listed cells
+// (L.wexc, then L.bexc).
 void launch_kick_drift(const Launch& L, hipStream_t s, float4* pos, float4* vel, const float4* acc,
-                       int64_t* dep, uint32_t* disp_slots, float dt);
+                       int64_t* dep, uint32_t* disp_slots, float dt, const uint32_t* keys,
+                       const uint32_t* cstart);
 void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A);
 void launch_momentum_kdk(const Launch& L, int S, hipStream_t s, const MomArgs& A);
 int brick_bits();

@@ -31,13 +31,24 @@ constexpr int TPB = 256;
 // ---------------------------------------------------------------------------------------------
 constexpr int KD_GRID = 2048;
 
+// How far a particle lies beyond its listed cell, in cells per axis: the distance of its scaled
+// coordinate from the cell's interval [c, c + 1], the periodic axes by minimum image (the cell
+// from its sort key, truncated and clamped as the key was).
+__device__ __forceinline__ float excursion(float xs, int c, float period, float half, bool periodic) {
+    float d = xs - ((float)c + 0.5f);
+    if (periodic) d += fabsf(d) > half ? copysignf(period, -d) : 0.f;
+    return fmaxf(fabsf(d) - 0.5f, 0.f);
+}
+
 __global__ __launch_bounds__(TPB) void k_kick_drift(const MemberDesc* __restrict__ mem,
                                                     const BlockMap* __restrict__ pmap, int npblk,
                                                     float4* __restrict__ pos, float4* __restrict__ vel,
                                                     const float4* __restrict__ acc,
                                                     int64_t* __restrict__ dep,
                                                     uint32_t* __restrict__ disp_slots, float hdt,
-                                                    float dt, const StepCtl* __restrict__ ctl) {
+                                                    float dt, const StepCtl* __restrict__ ctl,
+                                                    const uint32_t* __restrict__ keys,
+                                                    float4* __restrict__ wexc) {
     __shared__ float red[3][TPB / 64];
     if (ctl) {
         if (ctl->idle) return;
@@ -45,36 +56,59 @@ __global__ __launch_bounds__(TPB) void k_kick_drift(const MemberDesc* __restrict
         hdt = 0.5f * dt;
     }
     float mx = 0.f, my = 0.f, mz = 0.f;
-    for (int b = blockIdx.x; b < npblk; b += gridDim.x) {
+    for (int b = blockIdx.x; b < npblk; b += gridDim.x) {   // uniform over the workgroup
         const BlockMap bm = pmap[b];
         const MemberDesc& M = mem[bm.m];
         const DevParams& P = M.P;
         const int il = bm.base + threadIdx.x;
-        if (il >= P.N) continue;
-        const int i = M.p0 + il;
-        float4 v = vel[i];
-        float4 a = acc[i];
-        v.x = __builtin_fmaf(a.x, hdt, v.x);
-        v.y = __builtin_fmaf(a.y, hdt, v.y);
-        v.z = __builtin_fmaf(a.z, hdt, v.z);
-        float4 p = pos[i];
-        p.x = __builtin_fmaf(v.x, dt, p.x);
-        p.y = __builtin_fmaf(v.y, dt, p.y);
-        p.z = __builtin_fmaf(v.z, dt, p.z);
-        // periodic wrap (the reference's while loops, bounded so a non-finite position cannot spin)
-        for (int k = 0; k < 8 && p.x < 0.f; ++k) p.x += P.Lx;
-        for (int k = 0; k < 8 && p.x >= P.Lx; ++k) p.x -= P.Lx;
-        for (int k = 0; k < 8 && p.y < 0.f; ++k) p.y += P.Ly;
-        for (int k = 0; k < 8 && p.y >= P.Ly; ++k) p.y -= P.Ly;
-        if (p.z < 0.f) { p.z = -p.z; v.z = fabsf(v.z); }
-        if (p.z > P.Lz) { p.z = 2.f * P.Lz - p.z; v.z = -fabsf(v.z); }
-        pos[i] = p;
-        vel[i] = v;
-        dep[i] = 0;   // cells = particles per member: cell index = particle index
-        // drift bound of this step in cells per axis (|dx| <= |v dt|, also across a wrap or a wall)
-        mx = fmaxf(mx, fabsf(v.x * dt) * P.inv_dx);
-        my = fmaxf(my, fabsf(v.y * dt) * P.inv_dy);
-        mz = fmaxf(mz, fabsf(v.z * dt) * P.inv_dz);
+        float ex = 0.f, ey = 0.f, ez = 0.f;
+        if (il < P.N) {
+            const int i = M.p0 + il;
+            float4 v = vel[i];
+            float4 a = acc[i];
+            v.x = __builtin_fmaf(a.x, hdt, v.x);
+            v.y = __builtin_fmaf(a.y, hdt, v.y);
+            v.z = __builtin_fmaf(a.z, hdt, v.z);
+            float4 p = pos[i];
+            p.x = __builtin_fmaf(v.x, dt, p.x);
+            p.y = __builtin_fmaf(v.y, dt, p.y);
+            p.z = __builtin_fmaf(v.z, dt, p.z);
+            // periodic wrap (the reference's while loops, bounded so a non-finite position cannot spin)
+            for (int k = 0; k < 8 && p.x < 0.f; ++k) p.x += P.Lx;
+            for (int k = 0; k < 8 && p.x >= P.Lx; ++k) p.x -= P.Lx;
+            for (int k = 0; k < 8 && p.y < 0.f; ++k) p.y += P.Ly;
+            for (int k = 0; k < 8 && p.y >= P.Ly; ++k) p.y -= P.Ly;
+            if (p.z < 0.f) { p.z = -p.z; v.z = fabsf(v.z); }
+            if (p.z > P.Lz) { p.z = 2.f * P.Lz - p.z; v.z = -fabsf(v.z); }
+            pos[i] = p;
+            vel[i] = v;
+            dep[i] = 0;   // cells = particles per member: cell index = particle index
+            // drift bound of this step in cells per axis (|dx| <= |v dt|, also across a wrap or a wall)
+            mx = fmaxf(mx, fabsf(v.x * dt) * P.inv_dx);
+            my = fmaxf(my, fabsf(v.y * dt) * P.inv_dy);
+            mz = fmaxf(mz, fabsf(v.z * dt) * P.inv_dz);
+            if (keys) {
+                // beyond the listed cell (the key of the last sort), on the scaled coordinates the
+                // neighbour kernels difference; a non-finite position gives +inf or NaN, which no
+                // bound test accepts
+                const uint32_t kl = keys[i] & ((1u << P.end_bit) - 1u);
+                ex = excursion(p.x * P.inv_dx, (int)compact1by2(kl), P.fnx, P.hnx, true);
+                ey = excursion(p.y * P.inv_dy, (int)compact1by2(kl >> 1), P.fny, P.hny, true);
+                ez = excursion(p.z * P.inv_dz, (int)compact1by2(kl >> 2), 0.f, 0.f, false);
+                if (!(ex == ex && ey == ey && ez == ez)) ex = ey = ez = __int_as_float(0x7f800000);
+            }
+        }
+        if (keys) {
+            // wavefront maxima, one entry per wavefront of the block (member-local index / 64)
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) {
+                ex = fmaxf(ex, __shfl_xor(ex, o));
+                ey = fmaxf(ey, __shfl_xor(ey, o));
+                ez = fmaxf(ez, __shfl_xor(ez, o));
+            }
+            if ((threadIdx.x & 63) == 0 && il < P.N)
+                wexc[(size_t)b * (TPB / 64) + (threadIdx.x >> 6)] = make_float4(ex, ey, ez, 0.f);
+        }
     }
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -92,6 +126,45 @@ __global__ __launch_bounds__(TPB) void k_kick_drift(const MemberDesc* __restrict
         // non-negative floats order like their bit patterns
         atomicMax(&disp_slots[threadIdx.x * 64 + (blockIdx.x & 63)], __float_as_uint(m));
     }
+}
+
+// Per-brick drift bound, one wavefront per brick of the brick map: the largest excursion beyond
+// their listed cells of the particles listed in the brick (in key order one index range,
+// [cstart[key], cstart[key + 512])), from the wavefront maxima of the kick/drift kernel. An entry
+// covering particles of a neighbouring brick as well only raises the bound.
+__global__ __launch_bounds__(TPB) void k_brick_exc(const MemberDesc* __restrict__ mem,
+                                                   const BlockMap* __restrict__ bmap, int nbblk,
+                                                   const uint32_t* __restrict__ cstart,
+                                                   const float4* __restrict__ wexc, float4* __restrict__ bexc,
+                                                   const StepCtl* __restrict__ ctl) {
+    if (ctl && ctl->idle) return;
+    const int g = blockIdx.x * (TPB / 64) + (threadIdx.x >> 6);   // uniform over the wavefront
+    if (g >= nbblk) return;
+    const int lane = threadIdx.x & 63;
+    const BlockMap bm = bmap[g];
+    const MemberDesc& M = mem[bm.m];
+    const uint32_t key = M.k0 + (uint32_t)bm.base;
+    // the member's index range bounds the reads whatever the table holds
+    const uint32_t p0 = (uint32_t)M.p0, p1 = p0 + (uint32_t)M.P.N;
+    const uint32_t lo = min(max(cstart[key], p0), p1), hi = min(max(cstart[key + (1u << BRICK_BITS)], p0), p1);
+    float mx = 0.f, my = 0.f, mz = 0.f;
+    if (hi > lo) {
+        const float4* W = wexc + (size_t)M.first_block * (TPB / 64);
+        const uint32_t e1 = (hi - 1u - p0) >> 6;
+        for (uint32_t e = ((lo - p0) >> 6) + (uint32_t)lane; e <= e1; e += 64) {
+            const float4 v = W[e];
+            mx = fmaxf(mx, v.x);
+            my = fmaxf(my, v.y);
+            mz = fmaxf(mz, v.z);
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        mx = fmaxf(mx, __shfl_xor(mx, o));
+        my = fmaxf(my, __shfl_xor(my, o));
+        mz = fmaxf(mz, __shfl_xor(mz, o));
+    }
+    if (lane == 0) bexc[key >> BRICK_BITS] = make_float4(mx, my, mz, 0.f);
 }
 
 // Per-step bookkeeping, one wavefront: folds the drift maxima of the last step into the bound
@@ -475,10 +548,15 @@ void launch_step_begin(hipStream_t s, float* disp, uint32_t* disp_slots, int res
 }
 
 void launch_kick_drift(const Launch& L, hipStream_t s, float4* pos, float4* vel, const float4* acc,
-                       int64_t* dep, uint32_t* disp_slots, float dt) {
+                       int64_t* dep, uint32_t* disp_slots, float dt, const uint32_t* keys,
+                       const uint32_t* cstart) {
     const int grid = L.npblk < KD_GRID ? L.npblk : KD_GRID;
+    const bool per_brick = keys && cstart && L.wexc && L.bexc;
     k_kick_drift<<<grid, TPB, 0, s>>>(L.mem, L.pmap, L.npblk, pos, vel, acc, dep, disp_slots,
-                                      0.5f * dt, dt, L.ctl);
+                                      0.5f * dt, dt, L.ctl, per_brick ? keys : nullptr, L.wexc);
+    if (per_brick)
+        k_brick_exc<<<(L.nbblk + TPB / 64 - 1) / (TPB / 64), TPB, 0, s>>>(L.mem, L.bmap, L.nbblk, cstart, L.wexc,
+                                                                          L.bexc, L.ctl);
 }
 
 void launch_rain(const Launch& L, hipStream_t s, const int64_t* rain_in, const int64_t* dep,

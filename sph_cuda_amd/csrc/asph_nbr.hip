@@ -22,12 +22,14 @@
 // registers and walks, slab by slab, the rows of its stencil that lie in the staged slab, so the
 // candidate order is still the reference's. Only a brick with a single halo row beyond MAXP, more
 // than NSLAB slabs, or more than 65534 halo particles, runs the thread-per-particle loop over the
-// global cell table, as does mode B (drift since the last sort >= 0.95 cell, or a domain
-// narrower than a halo).
+// global cell table, as does mode B (a work item whose drift bound reaches 0.95 cell, or a
+// domain narrower than a halo).
 //
-// Candidate culling is exact: particles listed in cell c have moved at most delta cells (the
-// device-tracked maximum drift since the sort) so they lie within [c - delta, c + 1 + delta] on
-// each axis. Rows whose minimum metric distance exceeds the support, and cells of a row beyond
+// Candidate culling is exact: particles listed in cell c lie within [c - delta, c + 1 + delta] on
+// each axis, delta the work item's drift bound: the largest distance beyond their listed cells of
+// the particles listed in the bricks its halo can reach, measured by the kick/drift kernel of the
+// step (before its first step after a sort every particle is in its listed cell, and any recorded
+// bound holds). Rows whose minimum metric distance exceeds the support, and cells of a row beyond
 // the row's chord, cannot hold an in-support particle and are skipped; surviving candidates go
 // through the reference test r2 < 4h^2 unchanged.
 //
@@ -43,7 +45,7 @@ namespace asph {
 
 namespace {
 
-constexpr int NB = 9;              // Morton bits per brick: 8 x 8 x 8 cells
+constexpr int NB = BRICK_BITS;     // Morton bits per brick: 8 x 8 x 8 cells
 constexpr int BS = 8;              // brick edge in cells
 constexpr int BT = 512;            // threads per workgroup (one per brick cell)
 constexpr int BW = BT / 64;        // wavefronts per workgroup
@@ -434,6 +436,67 @@ __device__ __forceinline__ void prologue(const float* disp, const uint32_t* disp
                 ok = ok && (d < 0.95f);
             }
             sm.modeA = ok ? 1 : 0;
+        }
+    }
+    __syncthreads();
+}
+
+// The k-th of the bricks (by brick origin, in order along the axis) that hold the cells
+// [c0 - S - 1, c0 + BS + S + 1) of one axis, wrapped (periodic) or clipped (z): the cells of the
+// widest halo of brick c0. Returns how many there are (a brick repeated after a wrap counts again).
+template <int S>
+__device__ __forceinline__ int halo_brick(int c0, int n, bool periodic, int k, int& out) {
+    int cnt = 0, prev = -1;
+    out = c0;
+    for (int c = c0 - S - 1; c < c0 + BS + S + 1; ++c) {
+        int w = c;
+        if (periodic) w = wrapi(c, n);
+        else if (c < 0 || c >= n) continue;
+        const int bk = w & ~(BS - 1);
+        if (bk != prev) {
+            if (cnt == k) out = bk;
+            ++cnt;
+            prev = bk;
+        }
+    }
+    return cnt;
+}
+
+// The drift bound of one work item (brick kb of a member) and whether it allows mode A, from the
+// per-brick bounds: the largest excursion beyond their listed cells of the particles listed in
+// the bricks that hold the item's widest halo. Every particle the item owns, stages or walks is
+// listed there. Lane (ix, iy, iz) of the first wavefront reads the ix-th, iy-th and iz-th of those
+// bricks along each axis (at most 4 each; more, in a domain narrower than two bricks, gives mode B).
+// Without per-brick bounds, the member-wide bound since the sort (prologue).
+template <int S, bool MOM>
+__device__ __forceinline__ void item_prologue(const DevParams& P, uint32_t k0, uint32_t kb, const float4* bexc,
+                                              const float* disp, const uint32_t* disp_slots, Smem<S, MOM>& sm) {
+    if (!bexc) {
+        prologue<S, MOM>(disp, disp_slots, sm);
+        return;
+    }
+    const int tid = threadIdx.x;
+    if (tid < 64) {
+        int cx, cy, cz;
+        const int nbx = halo_brick<S>((int)compact1by2(kb), P.nx, true, tid & 3, cx);
+        const int nby = halo_brick<S>((int)compact1by2(kb >> 1), P.ny, true, (tid >> 2) & 3, cy);
+        const int nbz = halo_brick<S>((int)compact1by2(kb >> 2), P.nz, false, tid >> 4, cz);
+        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
+        if ((tid & 3) < nbx && ((tid >> 2) & 3) < nby && (tid >> 4) < nbz)
+            e = bexc[(k0 + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)) >> NB];
+        // NaN (a non-finite position) must not vanish in the maxima
+        if (!(e.x == e.x && e.y == e.y && e.z == e.z)) e.x = __int_as_float(0x7f800000);
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            e.x = fmaxf(e.x, __shfl_xor(e.x, o));
+            e.y = fmaxf(e.y, __shfl_xor(e.y, o));
+            e.z = fmaxf(e.z, __shfl_xor(e.z, o));
+        }
+        if (tid == 0) {
+            sm.delta[0] = e.x + DELTA_EPS;
+            sm.delta[1] = e.y + DELTA_EPS;
+            sm.delta[2] = e.z + DELTA_EPS;
+            sm.modeA = (nbx <= 4 && nby <= 4 && nbz <= 4 && e.x < 0.95f && e.y < 0.95f && e.z < 0.95f) ? 1 : 0;
         }
     }
     __syncthreads();
@@ -833,8 +896,9 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
     return b - j + (j % 8) * XCD_RUN + j / 8;
 }
 
-// Workgroup b plays up to two roles: mode B for brick b (members not in mode A), and mode A for
-// work item b (members in mode A). Every particle is computed by exactly one of them.
+// Workgroup b takes work item b: mode A (LDS halo) when the item's drift bound allows it and the
+// domain is wider than its halo, else mode B over the item's particles. Every listed particle
+// belongs to exactly one item.
 template <int S>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6))) void k_density_eos_brick(
         const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
@@ -854,47 +918,31 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
     if (b < n_members && tid < 3 * 64) A.maxv[b * 192 + tid] = 0u;   // max|v| slots of member b
-    prologue<S, false>(A.disp, A.disp_slots, sm);
-    if (b < nbblk) {  // mode B: particle per thread by index within the member
-        const BlockMap bm = bmap[b];
-        const MemberDesc& MD = mem[bm.m];
-        const DevParams& P = MD.P;
-        if (!member_mode_a<S, false>(P, sm)) {
-            count_wg(A.stats, ASPH_C_DENS_MODEB);
-            DensConst K;
-            K.p0 = MD.p0;
-            K.auto_f = auto_factor(P, A.dt);
-            sfc_params(P, A.clock ? A.clock[0] : A.simt, K.Ts, K.qs_sfc);
-            int il = bm.ord * BT + tid;
-            if (il < P.N) {
-                int i = MD.p0 + il;
-                float4 pi = A.pos[i];
-                float S1, S2;
-                dens_sum_global<S>(P, MD.k0, pi, A.pos, A.rho_old, A.cstart, S1, S2);
-                dens_epilogue(P, A, K, i, pi, S1, S2);
-            }
-        }
-    }
     const uint32_t nit = *nitems;
     if ((uint32_t)b >= nit) return;
     const int2 it = items[xcd_item((uint32_t)b, nit)];
     const BlockMap bm = bmap[it.x];
     const MemberDesc& MD = mem[bm.m];
     const DevParams& P = MD.P;
-    if (!member_mode_a<S, false>(P, sm)) return;
     const uint32_t k0 = MD.k0;
+    const uint32_t kb = (uint32_t)bm.base;
+    item_prologue<S, false>(P, k0, kb, A.bexc, A.disp, A.disp_slots, sm);
     DensConst K;
     K.p0 = MD.p0;
     K.auto_f = auto_factor(P, A.dt);
     sfc_params(P, A.clock ? A.clock[0] : A.simt, K.Ts, K.qs_sfc);
-    const uint32_t kb = (uint32_t)bm.base;
     uint32_t o0, o1;
     item_range(A.cstart, k0, kb, it.y, (uint32_t)csize, o0, o1);
     if (o0 >= o1) return;
-    const BrickGeo g = brick_geo<S, false>(P, sm, kb);
-    plan<S, false>(P, k0, g, A.cstart, sm);
-    if (sm.nslab == 0) {
-        count_wg(A.stats, ASPH_C_DENS_GLOBAL);
+    const bool modeA = member_mode_a<S, false>(P, sm);
+    if (modeA) {
+        const BrickGeo g = brick_geo<S, false>(P, sm, kb);
+        plan<S, false>(P, k0, g, A.cstart, sm);
+    }
+    if (!modeA || sm.nslab == 0) {
+        // mode B (the item's drift bound, or a domain narrower than its halo), or a halo that
+        // cannot be staged: thread per particle over the global cell table
+        count_wg(A.stats, modeA ? ASPH_C_DENS_GLOBAL : ASPH_C_DENS_MODEB);
         for (uint32_t i = o0 + tid; i < o1; i += BT) {
             float4 pi = A.pos[i];
             float S1, S2;
@@ -903,6 +951,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6))) void k_
         }
         return;
     }
+    const BrickGeo g = brick_geo<S, false>(P, sm, kb);
     count_wg(A.stats, sm.nslab == 1 ? ASPH_C_DENS_ONE_SLAB : ASPH_C_DENS_SLABBED);
     count_wg(A.stats, ASPH_C_DENS_STAGINGS, sm.nslab * (int)((o1 - o0 + BT - 1) / BT));
     const float* rho_old = A.rho_old;
@@ -1046,43 +1095,28 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     }
     const int tid = threadIdx.x;
     const int b = blockIdx.x;
-    prologue<S, true>(A.disp, A.disp_slots, sm);
-    if (b < nbblk) {
-        const BlockMap bm = bmap[b];
-        const MemberDesc& MD = mem[bm.m];
-        const DevParams& P = MD.P;
-        if (!member_mode_a<S, true>(P, sm)) {
-            count_wg(A.stats, ASPH_C_MOM_MODEB);
-            float mx = 0.f, my = 0.f, mz = 0.f;
-            int il = bm.ord * BT + tid;
-            if (il < P.N) {
-                int i = MD.p0 + il;
-                float4 pi = A.pos[i], vi = A.vel[i], ti = A.thq[i];
-                float rhoi = fmaxf(A.rho[i], 1e-3f), pri = A.prs[i];
-                MomAcc M = {0.f, 0.f, 0.f, 0.f, 0.f, 0.f};
-                mom_sum_global<S>(P, MD.k0, pi, vi, rhoi, pri, A.pos, A.vel, A.rho, A.prs, A.cstart, M);
-                mom_epilogue(P, A, i, pi, vi, ti, pri, M, mx, my, mz);
-            }
-            if (A.mode) reduce_maxv(A.maxv + bm.m * 192, mx, my, mz);
-        }
-    }
     const uint32_t nit = *nitems;
     if ((uint32_t)b >= nit) return;
     const int2 it = items[xcd_item((uint32_t)b, nit)];
     const BlockMap bm = bmap[it.x];
     const MemberDesc& MD = mem[bm.m];
     const DevParams& P = MD.P;
-    if (!member_mode_a<S, true>(P, sm)) return;
     const uint32_t k0 = MD.k0;
     const uint32_t kb = (uint32_t)bm.base;
+    item_prologue<S, true>(P, k0, kb, A.bexc, A.disp, A.disp_slots, sm);
     uint32_t o0, o1;
     item_range(A.cstart, k0, kb, it.y, (uint32_t)csize, o0, o1);
     if (o0 >= o1) return;
     float mx = 0.f, my = 0.f, mz = 0.f;
-    const BrickGeo g = brick_geo<S, true>(P, sm, kb);
-    plan<S, true>(P, k0, g, A.cstart, sm);
-    if (sm.nslab == 0) {
-        count_wg(A.stats, ASPH_C_MOM_GLOBAL);
+    const bool modeA = member_mode_a<S, true>(P, sm);
+    if (modeA) {
+        const BrickGeo g = brick_geo<S, true>(P, sm, kb);
+        plan<S, true>(P, k0, g, A.cstart, sm);
+    }
+    if (!modeA || sm.nslab == 0) {
+        // mode B (the item's drift bound, or a domain narrower than its halo), or a halo that
+        // cannot be staged: thread per particle over the global cell table
+        count_wg(A.stats, modeA ? ASPH_C_MOM_GLOBAL : ASPH_C_MOM_MODEB);
         for (uint32_t i = o0 + tid; i < o1; i += BT) {
             float4 pi = A.pos[i], vi = A.vel[i], ti = A.thq[i];
             float rhoi = fmaxf(A.rho[i], 1e-3f), pri = A.prs[i];
@@ -1091,6 +1125,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
             mom_epilogue(P, A, (int)i, pi, vi, ti, pri, M, mx, my, mz);
         }
     } else {
+        const BrickGeo g = brick_geo<S, true>(P, sm, kb);
         count_wg(A.stats, sm.nslab == 1 ? ASPH_C_MOM_ONE_SLAB : ASPH_C_MOM_SLABBED);
         count_wg(A.stats, ASPH_C_MOM_STAGINGS, sm.nslab * (int)((o1 - o0 + BT - 1) / BT));
         auto rec = [&](Smem<S, true>& s, uint32_t slot, uint32_t jg, float4 pj) {
@@ -1119,7 +1154,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // halo (margin min(S + 1, floor(1 + 2d + R))) therefore holds every partner: each thread walks the
 // halo cells within the margin of its listed cell, skipping cells whose box [c - d, c + 1 + d]
 // lies outside the support (nearest periodic image), and queues in-support partners with their
-// roles. Otherwise (d >= 1: mode B, by brick map as the forward), and for a halo that cannot be
+// roles. Otherwise (d >= 1: mode B, by brick map), and for a halo that cannot be
 // staged, each thread walks the global cell table within S + 1 + floor(d) cells.
 // The pair terms (asph_adjoint.h) and the sets of pairs are those of the forward; the sums are
 // float, in walk order.

@@ -282,6 +282,38 @@ def test_large_drift_without_sort():
         assert not bad, (k, errs)
 
 
+def test_local_fast_particles():
+    """A few particles of one brick moving more than a cell per step: only the work items whose
+    halo reaches that brick take the thread-per-particle path (the drift bound is kept per
+    brick), the rest stay on the LDS path within the same launch, and every step still matches
+    the oracle over the stale cell table."""
+    cfg, ic, dt = small_config("moist", (40, 40, 24))
+    st = s.ic_generate(cfg, ic)
+    cell = np.floor(st.pos[:, :3] / np.array([cfg.dx, cfg.dy, cfg.dz], np.float32)).astype(int)
+    fast = np.flatnonzero((cell[:, 0] < 8) & (cell[:, 1] < 8) & (cell[:, 2] >= 8) & (cell[:, 2] < 16))[:6]
+    st.vel[fast, 2] = np.float32(2.0 * cfg.dz / dt) * np.array([1, -1, 1, -1, 1, -1], np.float32)
+    print(f"[case] fast particles {fast.tolist()} at 2 cells per step in z", flush=True)
+    g, o = s.Sim(cfg), Oracle(cfg)
+    g.set_counters(True)
+    g.set_state(st)
+    o.set_state(st)
+    for x in (g, o):
+        x.sort()
+        x.force(dt)
+    g.counters()
+    for k in range(3):
+        for x in (g, o):
+            x.step(dt, k * dt, do_sort=False)
+        c = g.counters()
+        print(f"[case] step {k} counters {c}", flush=True)
+        bad, errs = state_close(g.get_state(), o.get_state(), TOL)
+        assert not bad, (k, errs)
+        assert np.array_equal(g.get_state().rho, o.get_state().rho), k
+        # 75 bricks: the 27 around the fast particles' brick take mode B, the others the LDS halo
+        assert c["dens_modeb"] > 0 and c["mom_modeb"] > 0, (k, c)
+        assert c["dens_one_slab"] + c["dens_slabbed"] > 0 and c["mom_one_slab"] + c["mom_slabbed"] > 0, (k, c)
+
+
 def test_water_budget_closes():
     """With the surface flux off, the water in the particles (vapour + cloud) plus the rain grid and
     the accumulated precipitation is conserved by the microphysics, sedimentation and transport:
