@@ -137,10 +137,8 @@ def controlled_run(sim, sim_end, report_every=0, wall_cap=0.0, counters=False):
                          trips=int(ev.trips), umax=[round(float(x), 2) for x in ev.umax],
                          kernel_ms_per_launch={k: round(v[0] / max(v[1], 1), 3) for k, v in kt.items()}))
         if counters:
-            # work items of the momentum launches by path since the last record
-            c = sim.counters()
-            rows[-1]["mom_items"] = {"lds": c["mom_one_slab"] + c["mom_slabbed"], "global": c["mom_global"],
-                                     "mode_b": c["mom_modeb"]}
+            # the neighbour kernels' work counters since the last record (include/asph.h)
+            rows[-1]["counters"] = sim.counters()
         log(f"[bench] {json.dumps(rows[-1])}")
         last[0], last[1] = time.perf_counter(), ev.step
         return bool(wall_cap) and (now - t0) > wall_cap

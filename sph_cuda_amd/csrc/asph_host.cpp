@@ -127,10 +127,11 @@ int create_impl(const asph_config* cfgs, int n, void* stream, asph_t** out) {
     rc |= h->alloc(&h->maxv, (size_t)192 * n);
     rc |= h->alloc(&h->disp, 4);
     rc |= h->alloc(&h->disp_slots, 3 * 64);
-    float4 *wexc = nullptr, *bexc = nullptr;
+    float4 *wexc = nullptr, *bexc = nullptr, *bhalo = nullptr;
     const size_t nbexc = (size_t)(h->nkeys >> brick_bits());
     rc |= h->alloc(&wexc, h->pmap_h.size() * (size_t)(particle_block_size() / 64));
     rc |= h->alloc(&bexc, nbexc);
+    rc |= h->alloc(&bhalo, h->bmap_h.size());
     rc |= h->alloc(&h->clock, 1);
     rc |= h->alloc(&h->audit_buf, 8);
     rc |= h->alloc(&h->mem_d, h->mem_h.size());
@@ -157,12 +158,13 @@ int create_impl(const asph_config* cfgs, int n, void* stream, asph_t** out) {
     if (e == hipSuccess) e = hipMemset(nitems, 0, sizeof(uint32_t));
     // per-brick drift bounds: zero holds for a state that has not moved since its sort
     if (e == hipSuccess) e = hipMemset(bexc, 0, nbexc * sizeof(float4));
+    if (e == hipSuccess) e = hipMemset(bhalo, 0, h->bmap_h.size() * sizeof(float4));
     if (e != hipSuccess) { asph_destroy(h); return fail("table upload: %s", hipGetErrorString(e)); }
     h->L.mem = h->mem_d; h->L.pmap = h->pmap_d; h->L.bmap = h->bmap_d;
     h->L.npblk = (int)h->pmap_h.size(); h->L.nbblk = (int)h->bmap_h.size(); h->L.n_members = n;
     h->L.items = items; h->L.nitems = nitems; h->L.item_hist = item_hist; h->L.item_meta = item_meta;
     h->L.nitem_max = nitem_max;
-    h->L.wexc = wexc; h->L.bexc = bexc;
+    h->L.wexc = wexc; h->L.bexc = bexc; h->L.bhalo = bhalo;
     h->L.chunk = 768;
     h->L.order = 1;
     h->L.gaps = gaps; h->L.ngaps = ngaps; h->L.gapcap = gapcap;
@@ -180,7 +182,7 @@ MomArgs mom_args(asph_t* h, float dt, int mode) {
     A.pos = h->pos; A.vel = h->vel; A.thq = h->thq; A.rho = h->rho; A.rinv = h->rinv; A.prs = h->prs;
     A.cstart = h->cstart; A.acc = h->acc; A.dthdt = h->dthdt; A.vel_out = h->vel2;
     A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
-    A.bexc = h->L.bexc;
+    A.bhalo = h->L.bhalo;
     A.stats = h->stats;
     A.av_out = h->rec_av;
     A.ctl = h->L.ctl;
@@ -220,6 +222,7 @@ void do_kick_drift(asph_t* h, float dt, int clock_mode, bool launch) {
         // the per-brick drift bound needs the listing of a sort (else the next sort resets it)
         launch_kick_drift(h->L, h->stream, h->pos, h->vel, h->acc, h->dep, h->disp_slots, dt,
                           h->sorted ? h->keys_sorted : nullptr, h->sorted ? h->cstart : nullptr);
+        if (h->sorted) launch_brick_halo(h->L, h->S, h->stream);
     }
     h->dep_pending = false;
 }
@@ -230,7 +233,7 @@ void do_density_eos(asph_t* h, float dt, double simt, int bits, bool use_clock, 
         A.pos = h->pos; A.rho_old = h->rho; A.rho_new = h->rho2; A.thq = h->thq; A.prs = h->prs;
         A.vel = h->vel; A.cstart = h->cstart; A.keys_sorted = h->keys_sorted; A.rain = h->rain;
         A.dep = h->dep; A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
-        A.bexc = h->L.bexc;
+        A.bhalo = h->L.bhalo;
         A.clock = use_clock ? h->clock : nullptr; A.simt = simt;
         if (h->L.ctl) A.clock = &h->L.ctl->simt;   // device controller: the step's start time
         A.ctl = h->L.ctl;
@@ -284,6 +287,7 @@ int reset_aux(asph_t* h, bool zero_acc) {
     HIPCHK(hipMemsetAsync(h->disp, 0, 4 * sizeof(float), h->stream));
     HIPCHK(hipMemsetAsync(h->disp_slots, 0, 3 * 64 * sizeof(uint32_t), h->stream));
     HIPCHK(hipMemsetAsync(h->L.bexc, 0, (size_t)(h->nkeys >> brick_bits()) * sizeof(float4), h->stream));
+    HIPCHK(hipMemsetAsync(h->L.bhalo, 0, h->bmap_h.size() * sizeof(float4), h->stream));
     HIPCHK(hipMemsetAsync(h->clock, 0, sizeof(double), h->stream));
     h->sorted = false;
     h->initialized = false;
@@ -1010,7 +1014,7 @@ DensArgs trial_dens_args(asph_t* h, float dt) {
     A.pos = h->pos; A.rho_old = h->rho; A.rho_new = h->rho2; A.thq = h->thq; A.prs = h->prs;
     A.vel = h->vel; A.cstart = h->cstart; A.keys_sorted = h->keys_sorted; A.rain = h->rain;
     A.dep = h->dep; A.maxv = h->maxv; A.disp = h->disp; A.disp_slots = h->disp_slots;
-    A.bexc = h->L.bexc;
+    A.bhalo = h->L.bhalo;
     A.clock = nullptr; A.simt = h->simt; A.stats = nullptr; A.s12_out = nullptr; A.rinv = h->rinv;
     A.dt = dt; A.bits = 0;
     return A;

@@ -47,6 +47,7 @@ struct Launch {
     // drift bound per brick: how far the particles listed in a brick lie beyond their listed cells
     float4* wexc = nullptr;  // [npblk * particle_block_size() / 64]: per-wavefront maxima (kick/drift)
     float4* bexc = nullptr;  // [nkeys >> brick_bits()]: per-brick maxima, indexed by key >> brick_bits()
+    float4* bhalo = nullptr; // [nbblk]: per brick of bmap, the maxima over the bricks its halo can reach
 };
 
 struct DensArgs {
@@ -63,7 +64,7 @@ struct DensArgs {
     uint32_t* maxv;              // [n_members][3][64], cleared here for the momentum kernel
     const float* disp;
     const uint32_t* disp_slots;
-    const float4* bexc = nullptr;   // per-brick drift bound (Launch::bexc), or null: the member-wide one
+    const float4* bhalo = nullptr;  // drift bound per brick of bmap (Launch::bhalo), or null: the member-wide one
     const double* clock;         // simulated time at the start of the step (device), or null:
     double simt;                 // then this value is used
     unsigned long long* stats;   // work counters (ASPH_C_DENS_*), or null
@@ -88,7 +89,7 @@ struct MomArgs {
     uint32_t* maxv;              // [n_members][3][64]
     const float* disp;
     const uint32_t* disp_slots;
-    const float4* bexc = nullptr;   // per-brick drift bound (Launch::bexc), or null: the member-wide one
+    const float4* bhalo = nullptr;  // drift bound per brick of bmap (Launch::bhalo), or null: the member-wide one
     unsigned long long* stats;   // work counters (ASPH_C_MOM_*), or null
     float4* av_out;              // adjoint tape: the viscous acceleration sums (x mm), or null
     const StepCtl* ctl = nullptr;   // device controller (dt, idle), or null
@@ -163,6 +164,8 @@ void launch_clamp_vel_ctl(const Launch& L, hipStream_t s, float4* vel);
 void launch_kick_drift(const Launch& L, hipStream_t s, float4* pos, float4* vel, const float4* acc,
                        int64_t* dep, uint32_t* disp_slots, float dt, const uint32_t* keys,
                        const uint32_t* cstart);
+// The drift bound of each brick's work items from the per-brick bounds (after launch_kick_drift).
+void launch_brick_halo(const Launch& L, int S, hipStream_t s);
 void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A);
 void launch_momentum_kdk(const Launch& L, int S, hipStream_t s, const MomArgs& A);
 int brick_bits();

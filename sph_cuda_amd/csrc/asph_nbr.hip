@@ -441,63 +441,22 @@ __device__ __forceinline__ void prologue(const float* disp, const uint32_t* disp
     __syncthreads();
 }
 
-// The k-th of the bricks (by brick origin, in order along the axis) that hold the cells
-// [c0 - S - 1, c0 + BS + S + 1) of one axis, wrapped (periodic) or clipped (z): the cells of the
-// widest halo of brick c0. Returns how many there are (a brick repeated after a wrap counts again).
-template <int S>
-__device__ __forceinline__ int halo_brick(int c0, int n, bool periodic, int k, int& out) {
-    int cnt = 0, prev = -1;
-    out = c0;
-    for (int c = c0 - S - 1; c < c0 + BS + S + 1; ++c) {
-        int w = c;
-        if (periodic) w = wrapi(c, n);
-        else if (c < 0 || c >= n) continue;
-        const int bk = w & ~(BS - 1);
-        if (bk != prev) {
-            if (cnt == k) out = bk;
-            ++cnt;
-            prev = bk;
-        }
-    }
-    return cnt;
-}
-
-// The drift bound of one work item (brick kb of a member) and whether it allows mode A, from the
-// per-brick bounds: the largest excursion beyond their listed cells of the particles listed in
-// the bricks that hold the item's widest halo. Every particle the item owns, stages or walks is
-// listed there. Lane (ix, iy, iz) of the first wavefront reads the ix-th, iy-th and iz-th of those
-// bricks along each axis (at most 4 each; more, in a domain narrower than two bricks, gives mode B).
-// Without per-brick bounds, the member-wide bound since the sort (prologue).
+// The drift bound of one work item and whether it allows mode A: its brick's entry of the per-brick
+// halo bounds (k_brick_halo), or without them the member-wide bound since the sort (prologue).
+// A non-finite entry (a non-finite position, or a domain narrower than two bricks) gives mode B.
 template <int S, bool MOM>
-__device__ __forceinline__ void item_prologue(const DevParams& P, uint32_t k0, uint32_t kb, const float4* bexc,
-                                              const float* disp, const uint32_t* disp_slots, Smem<S, MOM>& sm) {
-    if (!bexc) {
+__device__ __forceinline__ void item_prologue(const float4* bhalo, int brick, const float* disp,
+                                              const uint32_t* disp_slots, Smem<S, MOM>& sm) {
+    if (!bhalo) {
         prologue<S, MOM>(disp, disp_slots, sm);
         return;
     }
-    const int tid = threadIdx.x;
-    if (tid < 64) {
-        int cx, cy, cz;
-        const int nbx = halo_brick<S>((int)compact1by2(kb), P.nx, true, tid & 3, cx);
-        const int nby = halo_brick<S>((int)compact1by2(kb >> 1), P.ny, true, (tid >> 2) & 3, cy);
-        const int nbz = halo_brick<S>((int)compact1by2(kb >> 2), P.nz, false, tid >> 4, cz);
-        float4 e = make_float4(0.f, 0.f, 0.f, 0.f);
-        if ((tid & 3) < nbx && ((tid >> 2) & 3) < nby && (tid >> 4) < nbz)
-            e = bexc[(k0 + morton3((uint32_t)cx, (uint32_t)cy, (uint32_t)cz)) >> NB];
-        // NaN (a non-finite position) must not vanish in the maxima
-        if (!(e.x == e.x && e.y == e.y && e.z == e.z)) e.x = __int_as_float(0x7f800000);
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) {
-            e.x = fmaxf(e.x, __shfl_xor(e.x, o));
-            e.y = fmaxf(e.y, __shfl_xor(e.y, o));
-            e.z = fmaxf(e.z, __shfl_xor(e.z, o));
-        }
-        if (tid == 0) {
-            sm.delta[0] = e.x + DELTA_EPS;
-            sm.delta[1] = e.y + DELTA_EPS;
-            sm.delta[2] = e.z + DELTA_EPS;
-            sm.modeA = (nbx <= 4 && nby <= 4 && nbz <= 4 && e.x < 0.95f && e.y < 0.95f && e.z < 0.95f) ? 1 : 0;
-        }
+    if (threadIdx.x == 0) {
+        const float4 e = bhalo[brick];
+        sm.delta[0] = e.x + DELTA_EPS;
+        sm.delta[1] = e.y + DELTA_EPS;
+        sm.delta[2] = e.z + DELTA_EPS;
+        sm.modeA = (e.x < 0.95f && e.y < 0.95f && e.z < 0.95f) ? 1 : 0;
     }
     __syncthreads();
 }
@@ -926,7 +885,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6))) void k_
     const DevParams& P = MD.P;
     const uint32_t k0 = MD.k0;
     const uint32_t kb = (uint32_t)bm.base;
-    item_prologue<S, false>(P, k0, kb, A.bexc, A.disp, A.disp_slots, sm);
+    item_prologue<S, false>(A.bhalo, it.x, A.disp, A.disp_slots, sm);
     DensConst K;
     K.p0 = MD.p0;
     K.auto_f = auto_factor(P, A.dt);
@@ -1103,7 +1062,7 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
     const DevParams& P = MD.P;
     const uint32_t k0 = MD.k0;
     const uint32_t kb = (uint32_t)bm.base;
-    item_prologue<S, true>(P, k0, kb, A.bexc, A.disp, A.disp_slots, sm);
+    item_prologue<S, true>(A.bhalo, it.x, A.disp, A.disp_slots, sm);
     uint32_t o0, o1;
     item_range(A.cstart, k0, kb, it.y, (uint32_t)csize, o0, o1);
     if (o0 >= o1) return;
@@ -1665,6 +1624,86 @@ void launch_brick_items(const Launch& L, hipStream_t s, const uint32_t* cstart) 
                                     L.item_hist, L.ctl);
     k_items_scan<<<1, ITEM_BINS, 0, s>>>(L.item_hist, L.nitems, L.ctl);
     k_items_fill<<<g, 256, 0, s>>>(L.nbblk, L.item_meta, L.item_hist, L.items, L.ctl);
+}
+
+// ---------------------------------------------------------------------------------------------
+// Drift bound of each brick's work items: the largest per-brick bound (k_brick_exc) over the
+// bricks that hold the cells of the brick's widest halo, brick +- (S + 1) cells, wrapped in x and
+// y, clipped in z. Every particle an item of the brick owns, stages or walks is listed there. One
+// thread per brick of the brick map; a domain narrower than two bricks, where more than 4 bricks
+// per axis could hold the halo, gets +inf (mode B).
+// ---------------------------------------------------------------------------------------------
+// The bricks (origins) holding the cells [c0 - S - 1, c0 + BS + S + 1) of one axis, in order along
+// it; a brick met again after a wrap counts again. Returns how many there are; the first 4 land in
+// b0..b3.
+template <int S>
+__device__ __forceinline__ int halo_bricks(int c0, int n, bool periodic, int& b0, int& b1, int& b2, int& b3) {
+    int cnt = 0, prev = -1;
+    b0 = b1 = b2 = b3 = c0;
+#pragma unroll
+    for (int d = -S - 1; d < BS + S + 1; ++d) {
+        int c = c0 + d;
+        if (periodic) {
+            c = wrapi(c, n);
+        } else if (c < 0 || c >= n) {
+            continue;
+        }
+        const int bk = c & ~(BS - 1);
+        if (bk != prev) {
+            b0 = cnt == 0 ? bk : b0;
+            b1 = cnt == 1 ? bk : b1;
+            b2 = cnt == 2 ? bk : b2;
+            b3 = cnt == 3 ? bk : b3;
+            ++cnt;
+            prev = bk;
+        }
+    }
+    return cnt;
+}
+
+template <int S>
+__global__ __launch_bounds__(256) void k_brick_halo(const MemberDesc* __restrict__ mem,
+                                                    const BlockMap* __restrict__ bmap, int nbblk,
+                                                    const float4* __restrict__ bexc, float4* __restrict__ bhalo,
+                                                    const StepCtl* __restrict__ ctl) {
+    if (ctl && ctl->idle) return;
+    const int g = blockIdx.x * 256 + threadIdx.x;
+    if (g >= nbblk) return;
+    const BlockMap bm = bmap[g];
+    const MemberDesc& M = mem[bm.m];
+    const DevParams& P = M.P;
+    const uint32_t kb = (uint32_t)bm.base;
+    int x[4], y[4], z[4];
+    const int nx = halo_bricks<S>((int)compact1by2(kb), P.nx, true, x[0], x[1], x[2], x[3]);
+    const int ny = halo_bricks<S>((int)compact1by2(kb >> 1), P.ny, true, y[0], y[1], y[2], y[3]);
+    const int nz = halo_bricks<S>((int)compact1by2(kb >> 2), P.nz, false, z[0], z[1], z[2], z[3]);
+    float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
+    bool finite = true;
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+#pragma unroll
+            for (int c = 0; c < 4; ++c)
+                if (a < nz && b < ny && c < nx) {
+                    const float4 e = bexc[(M.k0 + morton3((uint32_t)x[c], (uint32_t)y[b], (uint32_t)z[a])) >> NB];
+                    finite = finite && e.x == e.x && e.y == e.y && e.z == e.z;
+                    m.x = fmaxf(m.x, e.x);
+                    m.y = fmaxf(m.y, e.y);
+                    m.z = fmaxf(m.z, e.z);
+                }
+    if (!finite || nx > 4 || ny > 4 || nz > 4) m.x = __int_as_float(0x7f800000);
+    bhalo[g] = m;
+}
+
+void launch_brick_halo(const Launch& L, int S, hipStream_t s) {
+    const int grid = (L.nbblk + 255) / 256;
+    switch (S) {
+        case 1: k_brick_halo<1><<<grid, 256, 0, s>>>(L.mem, L.bmap, L.nbblk, L.bexc, L.bhalo, L.ctl); break;
+        case 2: k_brick_halo<2><<<grid, 256, 0, s>>>(L.mem, L.bmap, L.nbblk, L.bexc, L.bhalo, L.ctl); break;
+        case 3: k_brick_halo<3><<<grid, 256, 0, s>>>(L.mem, L.bmap, L.nbblk, L.bexc, L.bhalo, L.ctl); break;
+        default: k_brick_halo<4><<<grid, 256, 0, s>>>(L.mem, L.bmap, L.nbblk, L.bexc, L.bhalo, L.ctl); break;
+    }
 }
 
 void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A) {
