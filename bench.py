@@ -49,8 +49,8 @@ PROFILED = {
     "window": "steps 6-25 of the 256^3 run",
     "valu_source": "profiles/step_256cubed_valu_counters.md",
     "traffic_source": "profiles/step_256cubed_hbm_traffic.md",
-    "valu_instr": dict(momentum_kdk=3.5800e9, density_eos=2.5903e9),
-    "traffic_bytes": dict(momentum_kdk=2.678e9, density_eos=2.213e9),
+    "valu_instr": dict(momentum_kdk=3.5841e9, density_eos=2.5977e9),
+    "traffic_bytes": dict(momentum_kdk=2.674e9, density_eos=2.210e9),
 }
 
 
