@@ -1630,8 +1630,10 @@ void launch_brick_items(const Launch& L, hipStream_t s, const uint32_t* cstart) 
 // Drift bound of each brick's work items: the largest per-brick bound (k_brick_exc) over the
 // bricks that hold the cells of the brick's widest halo, brick +- (S + 1) cells, wrapped in x and
 // y, clipped in z. Every particle an item of the brick owns, stages or walks is listed there. One
-// thread per brick of the brick map; a domain narrower than two bricks, where more than 4 bricks
-// per axis could hold the halo, gets +inf (mode B).
+// thread per brick of the brick map. Where nx and ny are multiples of the brick edge those are the
+// bricks at -BS, 0, +BS cells along each axis; otherwise they are enumerated cell by cell, and a
+// domain narrower than two bricks, where more than 4 per axis could hold the halo, gets +inf
+// (mode B).
 // ---------------------------------------------------------------------------------------------
 // The bricks (origins) holding the cells [c0 - S - 1, c0 + BS + S + 1) of one axis, in order along
 // it; a brick met again after a wrap counts again. Returns how many there are; the first 4 land in
@@ -1673,10 +1675,30 @@ __global__ __launch_bounds__(256) void k_brick_halo(const MemberDesc* __restrict
     const MemberDesc& M = mem[bm.m];
     const DevParams& P = M.P;
     const uint32_t kb = (uint32_t)bm.base;
+    const int bx0 = (int)compact1by2(kb), by0 = (int)compact1by2(kb >> 1), bz0 = (int)compact1by2(kb >> 2);
     int x[4], y[4], z[4];
-    const int nx = halo_bricks<S>((int)compact1by2(kb), P.nx, true, x[0], x[1], x[2], x[3]);
-    const int ny = halo_bricks<S>((int)compact1by2(kb >> 1), P.ny, true, y[0], y[1], y[2], y[3]);
-    const int nz = halo_bricks<S>((int)compact1by2(kb >> 2), P.nz, false, z[0], z[1], z[2], z[3]);
+    int nx, ny, nz;
+    if (((P.nx | P.ny) & (BS - 1)) == 0) {
+        // the bricks at -BS, 0, +BS cells along each axis (S + 1 < BS): wrapped in x and y, the
+        // ones inside [0, nz) in z
+        x[0] = bx0 - BS < 0 ? bx0 - BS + P.nx : bx0 - BS;
+        x[1] = bx0;
+        x[2] = bx0 + BS >= P.nx ? bx0 + BS - P.nx : bx0 + BS;
+        y[0] = by0 - BS < 0 ? by0 - BS + P.ny : by0 - BS;
+        y[1] = by0;
+        y[2] = by0 + BS >= P.ny ? by0 + BS - P.ny : by0 + BS;
+        const bool zlo = bz0 >= BS, zhi = bz0 + BS < P.nz;
+        z[0] = zlo ? bz0 - BS : bz0;
+        z[1] = zlo ? bz0 : bz0 + BS;
+        z[2] = bz0 + BS;
+        x[3] = y[3] = z[3] = 0;
+        nx = ny = 3;
+        nz = 1 + (zlo ? 1 : 0) + (zhi ? 1 : 0);
+    } else {
+        nx = halo_bricks<S>(bx0, P.nx, true, x[0], x[1], x[2], x[3]);
+        ny = halo_bricks<S>(by0, P.ny, true, y[0], y[1], y[2], y[3]);
+        nz = halo_bricks<S>(bz0, P.nz, false, z[0], z[1], z[2], z[3]);
+    }
     float4 m = make_float4(0.f, 0.f, 0.f, 0.f);
     bool finite = true;
 #pragma unroll
