@@ -186,7 +186,9 @@ int asph_set_state_device(asph_t* h, const float* pos4, const float* vel4, const
  * the last force evaluation. */
 int asph_get_state(asph_t* h, float* pos4, float* vel4, float* thq4, float* rho, float* acc4,
                    float* prs, float* dthdt);
-/* Device pointers of the current state buffers (valid until the next sort or step). */
+/* Device pointers of the current state buffers (valid until the next sort or step). Positions
+ * written through them count as a new state: run asph_sort before the neighbour stages (their
+ * drift bounds cover only the motion of asph_kick_drift since the last sort). */
 int asph_state_device_ptrs(asph_t* h, float** pos4, float** vel4, float** thq4, float** rho,
                            float** acc4);
 /* Sets acc (host float4 array), e.g. for the first kick after an external state upload. */
@@ -420,7 +422,7 @@ int asph_synchronize(asph_t* h);
 #define ASPH_C_DENS_SLABBED   1  /*          bricks whose halo was streamed in z-slabs        */
 #define ASPH_C_DENS_STAGINGS  2  /*          slab stagings (slabs x rounds of 512 particles) */
 #define ASPH_C_DENS_GLOBAL    3  /*          bricks that took the global-memory loop          */
-#define ASPH_C_DENS_MODEB     4  /*          workgroups of the thread-per-particle mode       */
+#define ASPH_C_DENS_MODEB     4  /*          work items of the thread-per-particle mode       */
 #define ASPH_C_DENS_CAND      5  /*          candidates tested after culling (LDS paths)      */
 #define ASPH_C_DENS_PAIRS     6  /*          in-support pairs evaluated (LDS paths)           */
 #define ASPH_C_MOM_ONE_SLAB   7  /* momentum: as 0..6                                          */
