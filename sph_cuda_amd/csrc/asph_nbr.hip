@@ -861,7 +861,7 @@ __device__ __forceinline__ uint32_t xcd_item(uint32_t b, uint32_t n) {
 template <int S>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(6))) void k_density_eos_brick(
         const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
-        const uint32_t* __restrict__ nitems, int nbblk, int n_members, int csize, DensArgs A0) {
+        const uint32_t* __restrict__ nitems, int n_members, int csize, DensArgs A0) {
     __shared__ Smem<S, false> sm;
     static_assert(sizeof(Smem<S, false>) <= HaloMax<S, false>::LDS, "density LDS budget");
     DensArgs A = A0;
@@ -1039,7 +1039,7 @@ __device__ __forceinline__ void reduce_maxv(uint32_t* maxv, float mx, float my, 
 template <int S>
 __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_momentum_kdk_brick(
         const MemberDesc* __restrict__ mem, const BlockMap* __restrict__ bmap, const int2* __restrict__ items,
-        const uint32_t* __restrict__ nitems, int nbblk, int csize, MomArgs A0) {
+        const uint32_t* __restrict__ nitems, int csize, MomArgs A0) {
     __shared__ Smem<S, true> sm;
     static_assert(sizeof(Smem<S, true>) <= HaloMax<S, true>::LDS, "momentum LDS budget");
     MomArgs A = A0;
@@ -1731,20 +1731,20 @@ void launch_brick_halo(const Launch& L, int S, hipStream_t s) {
 void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A) {
     dim3 g(L.nitem_max);
     switch (S) {
-        case 1: k_density_eos_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, L.chunk, A); break;
-        case 2: k_density_eos_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, L.chunk, A); break;
-        case 3: k_density_eos_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, L.chunk, A); break;
-        default: k_density_eos_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.n_members, L.chunk, A); break;
+        case 1: k_density_eos_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.n_members, L.chunk, A); break;
+        case 2: k_density_eos_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.n_members, L.chunk, A); break;
+        case 3: k_density_eos_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.n_members, L.chunk, A); break;
+        default: k_density_eos_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.n_members, L.chunk, A); break;
     }
 }
 
 void launch_momentum_kdk(const Launch& L, int S, hipStream_t s, const MomArgs& A) {
     dim3 g(L.nitem_max);
     switch (S) {
-        case 1: k_momentum_kdk_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, A); break;
-        case 2: k_momentum_kdk_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, A); break;
-        case 3: k_momentum_kdk_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, A); break;
-        default: k_momentum_kdk_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.nbblk, L.chunk, A); break;
+        case 1: k_momentum_kdk_brick<1><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.chunk, A); break;
+        case 2: k_momentum_kdk_brick<2><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.chunk, A); break;
+        case 3: k_momentum_kdk_brick<3><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.chunk, A); break;
+        default: k_momentum_kdk_brick<4><<<g, BT, 0, s>>>(L.mem, L.bmap, L.items, L.nitems, L.chunk, A); break;
     }
 }
 
