@@ -33,11 +33,13 @@
 // the row's chord, cannot hold an in-support particle and are skipped; surviving candidates go
 // through the reference test r2 < 4h^2 unchanged.
 //
-// The in-support pair work is separated from the candidate test by a per-thread queue of LDS
-// slots (QK entries): the test loop takes candidates two at a time and appends, and when a lane
-// of the wavefront could not take two more, all lanes drain theirs in order, several entries per
-// iteration with their operands loaded first. Lanes therefore spend the expensive pair body only
-// on in-support pairs, in the reference order.
+// In the momentum kernel the in-support pair work is separated from the candidate test by a
+// per-thread queue of LDS slots (QK entries): the test loop takes candidates two at a time and
+// appends, and when a lane of the wavefront could not take two more, all lanes drain theirs in
+// order, several entries per iteration with their operands loaded first. Lanes therefore spend the
+// expensive pair body only on in-support pairs, in the reference order. The density kernel's pair
+// term (a square root and the spline) is cheaper than queueing it: it is evaluated for every
+// candidate and added where the support test holds.
 #include "asph_adjoint.h"
 #include "asph_kernels.h"
 
@@ -50,11 +52,9 @@ constexpr int BS = 8;              // brick edge in cells
 constexpr int BT = 512;            // threads per workgroup (one per brick cell)
 constexpr int BW = BT / 64;        // wavefronts per workgroup
 constexpr int QK = 12;             // queued in-support pairs per thread, momentum
-constexpr int QKD = 6;             // and density (its halo slab shares a smaller LDS budget)
-constexpr int DU = 3;              // queue entries per drain iteration, density
-constexpr int MU = 3;              // and momentum
-static_assert(QKD % DU == 0 && QK % MU == 0 && QK % 2 == 0, "a drain iteration never reads past the queue");
-template <bool MOM> constexpr int queue_len() { return MOM ? QK : QKD; }
+constexpr int MU = 3;              // queue entries per drain iteration, momentum
+static_assert(QK % MU == 0 && QK % 2 == 0, "a drain iteration never reads past the queue");
+template <bool MOM> constexpr int queue_len() { return MOM ? QK : 0; }   // density queues nothing
 constexpr int NSLAB = 64;          // most slabs of one halo (beyond: the global-memory loop)
 constexpr int CHUNK_MIN = 256;     // bounds of the work-item size (tuned at run time)
 constexpr int CHUNK_MAX = 1024;
@@ -394,7 +394,7 @@ template <int S, bool MOM> struct Smem {
                                        // (density) or the particle's global index (momentum
                                        // and the reverse pair stages)
     uint16_t off[HaloMax<S, MOM>::HC + 1];   // exclusive start of each halo cell over the halo
-    uint16_t q[queue_len<MOM>()][BT];  // per-thread queue of LDS slots
+    uint16_t q[MOM ? queue_len<MOM>() : 1][MOM ? BT : 1];   // per-thread queue of LDS slots (momentum)
     uint32_t mk[3][HaloMax<S, MOM>::H];  // Morton bits of the halo's x, y, z cells (wrapped);
                                          // z: ~0u where the layer lies outside [0, nz)
     uint32_t wsum[BW];
@@ -759,44 +759,15 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
         const Cur cc = cur_cell<S>(P, g, pi);
         const float4 ci = cellpos(P, pi);
         float S1 = 0.f, S2 = 0.f;
-        int qn = 0, npair = 0, ncand = 0;
+        int npair = 0, ncand = 0;
         auto diff = [&](float4 cj, float& dxm, float& dym, float& dzm) { csep<WRAP>(P, ci, cj, dxm, dym, dzm); };
-        auto flush = [&]() {
-            // lanes past their queue evaluate a placeholder (slot 0 of the staged slab, which
-            // holds a particle whenever any lane has queued one) and drop it: no divergent region
-            // DU queue entries per iteration: all their staged positions are read before the
-            // pairs are evaluated (the sums still take them in queue order).
-            for (int k = 0; wave_any(k < qn); k += DU) {
-                bool act[DU];
-                float4 pj[DU];
-                uint32_t sl[DU];
-                // the slots are read unconditionally (k + u < QKD: k is a multiple of DU), so
-                // both reads are in flight together; a stale entry is replaced by slot 0
-#pragma unroll
-                for (int u = 0; u < DU; ++u) sl[u] = sm.q[k + u][tid];
-#pragma unroll
-                for (int u = 0; u < DU; ++u) {
-                    act[u] = k + u < qn;
-                    pj[u] = sm.p[act[u] ? sl[u] : 0u];
-                }
-#pragma unroll
-                for (int u = 0; u < DU; ++u) {
-                    if (u > 0 && !wave_any(act[u])) break;
-                    float dxm, dym, dzm;
-                    diff(pj[u], dxm, dym, dzm);
-                    float W = fker_sel(sqrt_rn(r2of(dxm, dym, dzm)) * P.inv_h);
-                    S1 = act[u] ? S1 + W : S1;
-                    S2 = act[u] ? S2 + W * pj[u].w : S2;
-                }
-            }
-            npair += qn;
-            qn = 0;
-        };
         for (int s = 0; s < ns; ++s) {
             if (ns > 1) stage_slab<S, false>(P, k0, g, s, A.pos, A.cstart, sm, rec);
             if (active) {
-                // candidates p and p + 1 (if two): both positions are read before either test;
-                // the queue takes them in order and is drained before it could overflow
+                // candidates p and p + 1 (if two), both staged positions read before either test.
+                // The kernel value is evaluated for every candidate (its spline is a few
+                // operations, cheaper than queueing the in-support ones) and added where the
+                // reference's test r2 < 4h^2 holds, in the reference order.
                 ncand += walk_rows<S, ROWS>(P, g, sm.delta, sm, s, cc, [&](int p, bool two) {
                     const float4 pj0 = sm.p[p], pj1 = sm.p[two ? p + 1 : p];
                     float dxm, dym, dzm;
@@ -804,16 +775,15 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
                     const float r20 = r2of(dxm, dym, dzm);
                     diff(pj1, dxm, dym, dzm);
                     const float r21 = r2of(dxm, dym, dzm);
-                    // both slots are written, and the count takes the accepted ones: a rejected
-                    // slot lies past the count and is overwritten (qn <= QKD - 2 here)
-                    sm.q[qn][tid] = (uint16_t)p;
-                    qn += r20 < P.lim ? 1 : 0;
-                    sm.q[qn][tid] = (uint16_t)(p + 1);
-                    qn += (two & (r21 < P.lim)) ? 1 : 0;   // & keeps both tests branch-free
-                    if (wave_any(qn > QKD - 2)) flush();
+                    const bool a0 = r20 < P.lim, a1 = two & (r21 < P.lim);
+                    const float W0 = fker_sel(sqrt_rn(r20) * P.inv_h), W1 = fker_sel(sqrt_rn(r21) * P.inv_h);
+                    S1 = a0 ? S1 + W0 : S1;
+                    S2 = a0 ? S2 + W0 * pj0.w : S2;
+                    S1 = a1 ? S1 + W1 : S1;
+                    S2 = a1 ? S2 + W1 * pj1.w : S2;
+                    npair += (a0 ? 1 : 0) + (a1 ? 1 : 0);
                 });
             }
-            flush();
         }
         if (active) dens_epilogue(P, A, K, (int)i, pi, S1, S2);
         if (A.stats) {
@@ -955,9 +925,10 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
         int qn = 0, npair = 0, ncand = 0;
         auto diff = [&](float4 cj, float& dxt, float& dyt, float& dzt) { csep<WRAP>(P, ci, cj, dxt, dyt, dzt); };
         auto flush = [&]() {
-            // lanes past their queue evaluate a placeholder (slot 0, as in the density kernel).
-            // Called only when some lane has queued a pair; the loop tests at the bottom, so the
-            // sums stay in place across iterations.
+            // lanes past their queue evaluate a placeholder (slot 0 of the staged slab, which
+            // holds a particle whenever any lane has queued one) with zeroed factors, so there is
+            // no divergent region. Called only when some lane has queued a pair; the loop tests
+            // at the bottom, so the sums stay in place across iterations.
             MomAcc L = M;
             int k = 0;
             // MU queue entries per iteration: the operands of all its pairs are loaded before they
@@ -967,8 +938,8 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
                 float4 pj[MU], vj[MU];   // vj = (vx, vy, vz, p'_j)
                 float2 rj[MU];           // (max(rho_j, 1e-3), its reciprocal)
                 uint32_t sl[MU];
-                // slots read unconditionally (k + u < QK: k is a multiple of MU), as in the
-                // density kernel
+                // slots read unconditionally (k + u < QK: k is a multiple of MU), so the reads are
+                // in flight together; a stale entry is replaced by slot 0
 #pragma unroll
                 for (int u = 0; u < MU; ++u) sl[u] = sm.q[k + u][tid];
 #pragma unroll
@@ -995,7 +966,8 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
         for (int s = 0; s < ns; ++s) {
             if (ns > 1) stage_slab<S, true>(P, k0, g, s, A.pos, A.cstart, sm, rec);
             if (active) {
-                // candidates p and p + 1 (if two), as in the density kernel
+                // candidates p and p + 1 (if two): both staged positions are read before either
+                // test; the queue takes them in order and is drained before it could overflow
                 ncand += walk_rows<S, ROWS>(P, g, sm.delta, sm, s, cc, [&](int p, bool two) {
                     const float4 pj0 = sm.p[p], pj1 = sm.p[two ? p + 1 : p];
                     float dxt, dyt, dzt;
@@ -1003,7 +975,9 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
                     const float r20 = r2of(dxt, dyt, dzt);
                     diff(pj1, dxt, dyt, dzt);
                     const float r21 = r2of(dxt, dyt, dzt);
-                    sm.q[qn][tid] = (uint16_t)p;   // as in the density kernel (qn <= QK - 2 here)
+                    // both slots are written and the count takes the accepted ones: a rejected slot
+                    // lies past the count and is overwritten (qn <= QK - 2 here)
+                    sm.q[qn][tid] = (uint16_t)p;
                     qn += ((r20 < P.lim) & (r20 >= 1e-12f)) ? 1 : 0;
                     sm.q[qn][tid] = (uint16_t)(p + 1);
                     qn += (two & (r21 < P.lim) & (r21 >= 1e-12f)) ? 1 : 0;
