@@ -53,6 +53,8 @@ constexpr int BT = 512;            // threads per workgroup (one per brick cell)
 constexpr int BW = BT / 64;        // wavefronts per workgroup
 constexpr int QK = 12;             // queued in-support pairs per thread, momentum
 constexpr int MU = 3;              // queue entries per drain iteration, momentum
+constexpr int DG = 1;              // candidates per step of the density walk (its pair terms are
+                                   // evaluated for every candidate, past a span's end too)
 static_assert(QK % MU == 0 && QK % 2 == 0, "a drain iteration never reads past the queue");
 template <bool MOM> constexpr int queue_len() { return MOM ? QK : 0; }   // density queues nothing
 constexpr int NSLAB = 64;          // most slabs of one halo (beyond: the global-memory loop)
@@ -676,7 +678,7 @@ __device__ __forceinline__ Cur cur_cell(const DevParams& P, const BrickGeo& g, f
 // every surviving candidate takes the exact support test. All lanes step through the same stencil
 // offsets together (rows of equal offset hold similar spans), skipping rows individually; rows
 // outside slab s are skipped the same way when the slab is part of the halo.
-template <int S, bool ROWS, class Cand, class Sm>
+template <int S, bool ROWS, int G = 2, class Cand, class Sm>
 __device__ __forceinline__ int walk_rows(const DevParams& P, const BrickGeo& g, const float* delta,
                                          const Sm& sm, int s, const Cur& c, Cand cand) {
     const float limc = P.lim * 1.0001f;
@@ -716,8 +718,8 @@ __device__ __forceinline__ int walk_rows(const DevParams& P, const BrickGeo& g, 
                 e = min(max(e, 0), cnt);
             }
             ncand += e - a;
-            // two candidates per step (the second absent at the end of an odd span)
-            for (int p = a; p < e; p += 2) cand(p, p + 1 < e);
+            // G candidates per step, fewer at the end of the span: cand(p, n), n = min(G, e - p)
+            for (int p = a; p < e; p += G) cand(p, e - p);
         }
     }
     return ncand;
@@ -764,24 +766,26 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
         for (int s = 0; s < ns; ++s) {
             if (ns > 1) stage_slab<S, false>(P, k0, g, s, A.pos, A.cstart, sm, rec);
             if (active) {
-                // candidates p and p + 1 (if two), both staged positions read before either test.
-                // The kernel value is evaluated for every candidate (its spline is a few
-                // operations, cheaper than queueing the in-support ones) and added where the
-                // reference's test r2 < 4h^2 holds, in the reference order.
-                ncand += walk_rows<S, ROWS>(P, g, sm.delta, sm, s, cc, [&](int p, bool two) {
-                    const float4 pj0 = sm.p[p], pj1 = sm.p[two ? p + 1 : p];
-                    float dxm, dym, dzm;
-                    diff(pj0, dxm, dym, dzm);
-                    const float r20 = r2of(dxm, dym, dzm);
-                    diff(pj1, dxm, dym, dzm);
-                    const float r21 = r2of(dxm, dym, dzm);
-                    const bool a0 = r20 < P.lim, a1 = two & (r21 < P.lim);
-                    const float W0 = fker_sel(sqrt_rn(r20) * P.inv_h), W1 = fker_sel(sqrt_rn(r21) * P.inv_h);
-                    S1 = a0 ? S1 + W0 : S1;
-                    S2 = a0 ? S2 + W0 * pj0.w : S2;
-                    S1 = a1 ? S1 + W1 : S1;
-                    S2 = a1 ? S2 + W1 * pj1.w : S2;
-                    npair += (a0 ? 1 : 0) + (a1 ? 1 : 0);
+                // one candidate per step (DG). The kernel value is evaluated for every candidate
+                // (its spline is a few operations, cheaper than queueing the in-support ones) and
+                // added where the reference's test r2 < 4h^2 holds, in the reference order; with
+                // more candidates per step, the evaluations past the end of odd spans cost more
+                // than the overlap gains.
+                ncand += walk_rows<S, ROWS, DG>(P, g, sm.delta, sm, s, cc, [&](int p, int n) {
+                    float4 pj[DG];
+#pragma unroll
+                    for (int u = 0; u < DG; ++u) pj[u] = sm.p[u < n ? p + u : p];
+#pragma unroll
+                    for (int u = 0; u < DG; ++u) {
+                        float dxm, dym, dzm;
+                        diff(pj[u], dxm, dym, dzm);
+                        const float r2 = r2of(dxm, dym, dzm);
+                        const bool a = (u < n) & (r2 < P.lim);
+                        const float W = fker_sel(sqrt_rn(r2) * P.inv_h);
+                        S1 = a ? S1 + W : S1;
+                        S2 = a ? S2 + W * pj[u].w : S2;
+                        npair += a ? 1 : 0;
+                    }
                 });
             }
         }
@@ -968,7 +972,8 @@ __device__ __forceinline__ void mom_brick(const DevParams& P, const MomArgs& A, 
             if (active) {
                 // candidates p and p + 1 (if two): both staged positions are read before either
                 // test; the queue takes them in order and is drained before it could overflow
-                ncand += walk_rows<S, ROWS>(P, g, sm.delta, sm, s, cc, [&](int p, bool two) {
+                ncand += walk_rows<S, ROWS>(P, g, sm.delta, sm, s, cc, [&](int p, int n) {
+                    const bool two = n > 1;
                     const float4 pj0 = sm.p[p], pj1 = sm.p[two ? p + 1 : p];
                     float dxt, dyt, dzt;
                     diff(pj0, dxt, dyt, dzt);
