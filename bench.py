@@ -445,6 +445,7 @@ def c2_bench(steps=600):
     sim.sort()
     sim.force(dt)
     tune = sim.autotune(reps=2)
+    log(f"[bench] c2 autotune: {tune['best']} of {tune['trials']}")
     sim.set_profiling(True)
     sim.kernel_times()
     sim.synchronize()
@@ -479,6 +480,7 @@ def sustained_bench(t_end=60.0):
     sim.sort()
     sim.force(dt)
     tune = sim.autotune(reps=2)
+    log(f"[bench] sustained autotune: {tune['best']} of {tune['trials']}")
     sim.set_profiling(True)
     sim.kernel_times()
     rc, ts, wall, _ = controlled_run(sim, t_end)

@@ -393,9 +393,11 @@ typedef struct asph_tune_report {
 int asph_set_tuning(asph_t* h, const asph_tuning* t);
 int asph_get_tuning(asph_t* h, asph_tuning* t);
 /* Times the two neighbour kernels for every candidate shape on the current sorted,
- * force-evaluated state (reps launches each, after one warm-up) and keeps the fastest; among
- * the heaviest-first candidates within 3% of it, it keeps instead the largest chunk that gives at
- * least two work items per CU, or the smallest chunk when none does. Reports all. The state
+ * force-evaluated state (reps launches each, after one warm-up). It keeps the heaviest-first
+ * order unless key order beats all its candidates by more than 10%, and within the order kept the
+ * fastest candidate; among the heaviest-first candidates within 3% of their fastest, it keeps
+ * instead the largest chunk that gives at least two work items per CU, or the smallest chunk when
+ * none does. Reports all. The state
  * (including acc, dthdt and max|v|) is unchanged. Allocates scratch for the duration. */
 int asph_autotune(asph_t* h, int32_t reps, asph_tune_report* rep);
 

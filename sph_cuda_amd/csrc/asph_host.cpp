@@ -142,7 +142,7 @@ int create_impl(const asph_config* cfgs, int n, void* stream, asph_t** out) {
     uint32_t *nitems = nullptr, *item_hist = nullptr, *item_meta = nullptr;
     rc |= h->alloc(&items, (size_t)nitem_max);
     rc |= h->alloc(&nitems, 1);
-    rc |= h->alloc(&item_hist, (size_t)2 * brick_item_bins());
+    rc |= h->alloc(&item_hist, brick_item_hist_words((int)h->bmap_h.size()));
     rc |= h->alloc(&item_meta, h->bmap_h.size());
     const uint32_t gapcap = cell_gap_capacity(h->nkeys);
     uint4* gaps = nullptr;
@@ -153,8 +153,7 @@ int create_impl(const asph_config* cfgs, int n, void* stream, asph_t** out) {
     hipError_t e = hipMemcpy(h->mem_d, h->mem_h.data(), h->mem_h.size() * sizeof(MemberDesc), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->pmap_d, h->pmap_h.data(), h->pmap_h.size() * sizeof(BlockMap), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(h->bmap_d, h->bmap_h.data(), h->bmap_h.size() * sizeof(BlockMap), hipMemcpyHostToDevice);
-    // the item histogram must start at zero (each build clears it again); no items before a sort
-    if (e == hipSuccess) e = hipMemset(item_hist, 0, (size_t)2 * brick_item_bins() * sizeof(uint32_t));
+    // no items before a sort
     if (e == hipSuccess) e = hipMemset(nitems, 0, sizeof(uint32_t));
     // per-brick drift bounds: zero holds for a state that has not moved since its sort
     if (e == hipSuccess) e = hipMemset(bexc, 0, nbexc * sizeof(float4));
@@ -1098,17 +1097,23 @@ int asph_autotune(asph_t* h, int32_t reps, asph_tune_report* rep) {
         if (e == hipSuccess) e = hipMemcpy(&nit[k], h->L.nitems, sizeof(uint32_t), hipMemcpyDeviceToHost);
         ms[k] = (double)t / reps;
     }
-    // Fastest shape, except that among the heaviest-first shapes within 3% of it the largest chunk
-    // that still gives two workgroups per CU is kept (or, when none does, the smallest chunk). On a
-    // near-uniform state the shapes tie: in a large domain, large chunks and the heaviest-first
-    // order then protect the launch from the dense regions that develop later (the long
-    // workgroups start first, and fewer items restage a dense brick's halo); in a small one,
-    // a launch is as long as one workgroup's chain, and small chunks shorten it.
-    size_t best = 0;
+    // The heaviest-first order unless brick order is more than 10% faster than every
+    // heaviest-first shape; within the chosen order the fastest shape, except that among the
+    // heaviest-first shapes within 3% of their fastest the largest chunk that still gives two
+    // workgroups per CU is kept (or, when none does, the smallest chunk). On a near-uniform state
+    // the shapes tie: in a large domain, large chunks and the heaviest-first order then protect
+    // the launch from the dense regions that develop later (the long workgroups start first, and
+    // fewer items restage a dense brick's halo; brick order, chosen at t = 0 of the 128^3 run for
+    // being more than 3% faster there, made its 600 steps 29% slower); in a small one, a launch
+    // is as long as one workgroup's chain, and small chunks shorten it.
+    size_t best = 0, best1 = cand.size();
     for (size_t k = 1; k < cand.size(); ++k)
         if (ms[k] < ms[best]) best = k;
+    for (size_t k = 0; k < cand.size(); ++k)
+        if (cand[k].order == 1 && (best1 == cand.size() || ms[k] < ms[best1])) best1 = k;
+    if (best1 < cand.size() && ms[best1] <= 1.10 * ms[best]) best = best1;
     size_t keep1 = cand.size(), small1 = cand.size();
-    for (size_t k = 0; k < cand.size(); ++k) {
+    for (size_t k = 0; k < cand.size() && cand[best].order == 1; ++k) {
         if (cand[k].order != 1 || ms[k] > 1.03 * ms[best]) continue;
         if (nit[k] >= 2u * (uint32_t)ncu && (keep1 == cand.size() || cand[k].chunk > cand[keep1].chunk)) keep1 = k;
         if (small1 == cand.size() || cand[k].chunk < cand[small1].chunk) small1 = k;

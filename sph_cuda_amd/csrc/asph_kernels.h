@@ -34,7 +34,7 @@ struct Launch {
     // work items of the brick kernels, rebuilt after every sort (launch_brick_items)
     int2* items;             // [nitem_max]: (brick, chunk), heaviest first
     uint32_t* nitems;        // number of valid items
-    uint32_t* item_hist;     // [2 * brick_item_bins()] counting-sort scratch (zero between builds)
+    uint32_t* item_hist;     // [brick_item_hist_words(nbblk)] counting-sort scratch
     uint32_t* item_meta;     // [nbblk] scratch
     int nitem_max;           // grid of the brick kernels (bound for the smallest chunk)
     int chunk;               // listed particles per work item, at most (tuning)
@@ -169,7 +169,7 @@ void launch_brick_halo(const Launch& L, int S, hipStream_t s);
 void launch_density_eos(const Launch& L, int S, hipStream_t s, const DensArgs& A);
 void launch_momentum_kdk(const Launch& L, int S, hipStream_t s, const MomArgs& A);
 int brick_bits();
-int brick_item_bins();
+size_t brick_item_hist_words(int nbblk);
 int brick_item_capacity(int nbblk, int64_t n);
 int brick_chunk_min();
 int brick_chunk_max();

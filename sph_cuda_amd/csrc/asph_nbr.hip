@@ -1506,12 +1506,15 @@ __global__ __launch_bounds__(BT) __attribute__((amdgpu_waves_per_eu(4))) void k_
 // Work items of the brick kernels: every non-empty brick is split into chunks of at most `csize`
 // listed particles, and (order = 1) the chunks are ordered by the brick's estimated cost (chunk
 // size x brick occupancy, 4 bins per octave), heaviest first, so that the long workgroups of dense
-// regions start early instead of forming the tail of the launch. Counting sort: count, scan, fill.
+// regions start early instead of forming the tail of the launch. Stable counting sort (count,
+// scan, fill): within a bin, and for order = 0 over the whole map, the items keep the brick order
+// (Morton order of each member), so that consecutive items, which run on one XCD, share halos.
 // ---------------------------------------------------------------------------------------------
 constexpr int ITEM_BINS = 128;
+constexpr int ITEM_SEGS = 8;    // segments of the count workgroups per bin in k_items_scan
 
-// Per-brick item count and cost bin; the bin histogram is aggregated in LDS first (one global
-// atomic per bin and workgroup: most bricks share a bin).
+// Per-brick item count and cost bin, and the items of each bin in this workgroup's 256 bricks
+// (hist[blockIdx.x][bin], every bin written).
 __global__ __launch_bounds__(256) void k_items_count(const MemberDesc* __restrict__ mem,
                                                      const BlockMap* __restrict__ bmap, int nbblk,
                                                      uint32_t csize, int order,
@@ -1541,58 +1544,77 @@ __global__ __launch_bounds__(256) void k_items_count(const MemberDesc* __restric
         meta[b] = (ni << 8) | (uint32_t)bin;
     }
     __syncthreads();
-    if (t < ITEM_BINS && lh[t]) atomicAdd(&hist[t], lh[t]);
+    if (t < ITEM_BINS) hist[(size_t)blockIdx.x * ITEM_BINS + t] = lh[t];
 }
 
-// One workgroup of ITEM_BINS threads: bin starts (heaviest bin first) into hist[ITEM_BINS + k],
-// the item count, and the counts cleared for the next build.
-__global__ __launch_bounds__(ITEM_BINS) void k_items_scan(uint32_t* __restrict__ hist, uint32_t* __restrict__ nitems,
-                                                          const StepCtl* __restrict__ ctl) {
+// One workgroup: thread (segment j, bin k) sums bin k over its segment of the g count workgroups;
+// the bins are scanned (heaviest first) and the segments within each bin, and each count is then
+// replaced by the position of the workgroup's first item of that bin. Sets the item count.
+__global__ __launch_bounds__(ITEM_BINS * ITEM_SEGS) void k_items_scan(uint32_t* __restrict__ hist, int g,
+                                                                      uint32_t* __restrict__ nitems,
+                                                                      const StepCtl* __restrict__ ctl) {
     if (ctl && !ctl->do_sort) return;
+    __shared__ uint32_t seg[ITEM_SEGS][ITEM_BINS];
     __shared__ uint32_t v[ITEM_BINS];
-    const int t = threadIdx.x;
-    v[t] = hist[t];
-    hist[t] = 0u;
+    const int k = threadIdx.x % ITEM_BINS, j = threadIdx.x / ITEM_BINS;
+    const int per = (g + ITEM_SEGS - 1) / ITEM_SEGS;
+    const int w0 = min(g, j * per), w1 = min(g, w0 + per);
+    uint32_t tot = 0u;
+#pragma unroll 8
+    for (int w = w0; w < w1; ++w) tot += hist[(size_t)w * ITEM_BINS + k];
+    seg[j][k] = tot;
     __syncthreads();
-    for (int o = 1; o < ITEM_BINS; o <<= 1) {
-        uint32_t add = t >= o ? v[t - o] : 0u;
+    if (j == 0) {
+        uint32_t bin_tot = 0u;
+        for (int q = 0; q < ITEM_SEGS; ++q) bin_tot += seg[q][k];
+        v[k] = bin_tot;
+    }
+    __syncthreads();
+    for (int o = 1; o < ITEM_BINS; o <<= 1) {   // inclusive scan of the bin totals
+        uint32_t add = (j == 0 && k >= o) ? v[k - o] : 0u;
         __syncthreads();
-        v[t] += add;
+        if (j == 0) v[k] += add;
         __syncthreads();
     }
-    hist[ITEM_BINS + t] = t ? v[t - 1] : 0u;
-    if (t == ITEM_BINS - 1) *nitems = v[t];
+    uint32_t run = k ? v[k - 1] : 0u;
+    for (int q = 0; q < j; ++q) run += seg[q][k];
+#pragma unroll 8
+    for (int w = w0; w < w1; ++w) {
+        const size_t a = (size_t)w * ITEM_BINS + k;
+        const uint32_t c = hist[a];
+        hist[a] = run;
+        run += c;
+    }
+    if (threadIdx.x == ITEM_BINS - 1) *nitems = v[ITEM_BINS - 1];
 }
 
-// Writes each brick's items at its bin's cursor: positions are reserved per workgroup and bin
-// with one global atomic, and handed out within the workgroup through LDS atomics.
+// Writes each brick's items at its workgroup's position in its bin plus the items of the same bin
+// from the lower bricks of the workgroup.
 __global__ __launch_bounds__(256) void k_items_fill(int nbblk, const uint32_t* __restrict__ meta,
-                                                    uint32_t* __restrict__ hist, int2* __restrict__ items,
+                                                    const uint32_t* __restrict__ hist, int2* __restrict__ items,
                                                     const StepCtl* __restrict__ ctl) {
     if (ctl && !ctl->do_sort) return;
-    __shared__ uint32_t lh[ITEM_BINS], base[ITEM_BINS];
+    __shared__ uint32_t lm[256];
     const int t = threadIdx.x;
-    if (t < ITEM_BINS) lh[t] = 0u;
-    __syncthreads();
     const int b = blockIdx.x * 256 + t;
-    uint32_t m = 0u, local = 0u;
-    if (b < nbblk) {
-        m = meta[b];
-        if (m >> 8) local = atomicAdd(&lh[m & 0xffu], m >> 8);
-    }
-    __syncthreads();
-    if (t < ITEM_BINS && lh[t]) base[t] = atomicAdd(&hist[ITEM_BINS + t], lh[t]);
+    const uint32_t m = b < nbblk ? meta[b] : 0u;
+    lm[t] = m;
     __syncthreads();
     const uint32_t ni = m >> 8;
     if (!ni) return;
-    const uint32_t pos = base[m & 0xffu] + local;
+    const uint32_t bin = m & 0xffu;
+    uint32_t pos = hist[(size_t)blockIdx.x * ITEM_BINS + bin];
+    for (int q = 0; q < t; ++q) {
+        const uint32_t o = lm[q];
+        pos += (o & 0xffu) == bin ? o >> 8 : 0u;
+    }
     for (uint32_t k = 0; k < ni; ++k) items[pos + k] = make_int2(b, (int)k);
 }
 
 }  // namespace
 
 int brick_bits() { return NB; }
-int brick_item_bins() { return ITEM_BINS; }
+size_t brick_item_hist_words(int nbblk) { return (size_t)((nbblk + 255) / 256) * ITEM_BINS; }
 int brick_item_capacity(int nbblk, int64_t n) { return nbblk + (int)((n + CHUNK_MIN - 1) / CHUNK_MIN); }
 int brick_chunk_min() { return CHUNK_MIN; }
 int brick_chunk_max() { return CHUNK_MAX; }
@@ -1601,7 +1623,7 @@ void launch_brick_items(const Launch& L, hipStream_t s, const uint32_t* cstart) 
     const int g = (L.nbblk + 255) / 256;
     k_items_count<<<g, 256, 0, s>>>(L.mem, L.bmap, L.nbblk, (uint32_t)L.chunk, L.order, cstart, L.item_meta,
                                     L.item_hist, L.ctl);
-    k_items_scan<<<1, ITEM_BINS, 0, s>>>(L.item_hist, L.nitems, L.ctl);
+    k_items_scan<<<1, ITEM_BINS * ITEM_SEGS, 0, s>>>(L.item_hist, g, L.nitems, L.ctl);
     k_items_fill<<<g, 256, 0, s>>>(L.nbblk, L.item_meta, L.item_hist, L.items, L.ctl);
 }
 

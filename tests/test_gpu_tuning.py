@@ -57,15 +57,15 @@ def test_autotune_keeps_state_and_steps():
     print(f"[case] autotune {rep}", flush=True)
     assert len(rep["trials"]) == 10 and all(t[2] > 0 for t in rep["trials"])
     assert b.tuning() == rep["best"]
-    # the rule: among the heaviest-first shapes within 3% of the fastest, the largest chunk with
-    # two work items per CU, or the smallest when none has; the fastest when none is that close
+    # the rule: the heaviest-first order unless brick order beats all its shapes by more than
+    # 10%; then, among the heaviest-first shapes within 3% of their fastest, the largest chunk with
+    # two work items per CU, or the smallest when none has
     fastest = min(t[2] for t in rep["trials"])
-    near = [t for t in rep["trials"] if t[1] == 1 and t[2] <= 1.03 * fastest]
-    full = [t for t in near if t[3] >= 2 * rep["cus"]]
-    if full:
-        want = max(full, key=lambda t: t[0])
-    elif near:
-        want = min(near, key=lambda t: t[0])
+    fastest1 = min(t[2] for t in rep["trials"] if t[1] == 1)
+    if fastest1 <= 1.10 * fastest:
+        near = [t for t in rep["trials"] if t[1] == 1 and t[2] <= 1.03 * fastest1]
+        full = [t for t in near if t[3] >= 2 * rep["cus"]]
+        want = max(full, key=lambda t: t[0]) if full else min(near, key=lambda t: t[0])
     else:
         want = min(rep["trials"], key=lambda t: t[2])
     assert (rep["best"]["chunk"], rep["best"]["order"]) == (want[0], want[1])
