@@ -49,8 +49,8 @@ PROFILED = {
     "window": "steps 6-25 of the 256^3 run",
     "valu_source": "profiles/step_256cubed_valu_counters.md",
     "traffic_source": "profiles/step_256cubed_hbm_traffic.md",
-    "valu_instr": dict(momentum_kdk=3.5841e9, density_eos=2.5977e9),
-    "traffic_bytes": dict(momentum_kdk=2.674e9, density_eos=2.210e9),
+    "valu_instr": dict(momentum_kdk=3.5939e9, density_eos=2.5804e9),
+    "traffic_bytes": dict(momentum_kdk=2.675e9, density_eos=2.145e9),
 }
 
 
@@ -264,14 +264,15 @@ def ensemble_bench(n_members, steps, seed=5, sequential=True):
 
 
 def valu_probe():
-    """The device's VALU issue roofline (asph_probe_valu): independent non-packed FP32 FMA chains.
-    The neighbour kernels' own instruction counts come from SQ counters (profiles/)."""
+    """The device's VALU issue roofline (asph_probe_valu): independent non-packed FP32 FMA chains,
+    the faster of the two-operand (v_fmac_f32) and three-operand (v_fma_f32) encodings. The
+    neighbour kernels' own instruction counts come from SQ counters (profiles/)."""
     import ctypes as C
 
     from sph_cuda_amd import _lib as L
 
     w = C.c_double()
-    L.check(L.load().asph_probe_valu(8192, 5, C.byref(w)), "asph_probe_valu")
+    L.check(L.load().asph_probe_valu(4096, 3, C.byref(w)), "asph_probe_valu")
     return {"wave_fma_per_s": w.value, "fp32_nonpacked_tflops": w.value * 128 / 1e12,
             "wave_fma_per_simd_cycle_at_2p4GHz": w.value / (2.4e9 * 1024)}
 
@@ -641,7 +642,8 @@ def main():
                          "the probe's non-packed FMA rate (not the 157.3 TFLOP/s packed peak)",
             "frac": valu_achieved / peak_valu if valu_achieved else None,
             "valu_wave_instr_per_s": valu_achieved * 1e9 if valu_achieved else None,
-            "peak_source": "valu_probe below: independent non-packed FP32 FMA chains, this run",
+            "peak_source": "valu_probe below: independent non-packed FP32 FMA chains (the faster of the "
+                           "v_fmac_f32 and v_fma_f32 encodings), this run",
             "valu_instr_per_launch": instr,
             "valu_instr_source": f"{PROFILED['valu_source']} (SQ_INSTS_VALU, {PROFILED['window']})"
                                  if instr else None,

@@ -364,9 +364,10 @@ int asph_adjoint_rain(asph_t* h, const asph_adjoint_args* a, float* dJ_dtheta0, 
 /* --- measurement -------------------------------------------------------------------------- */
 
 /* The VALU issue roofline of the current device (no reference counterpart; measurement only):
- * independent non-packed FP32 FMA chains over 32 wavefronts per CU, `iters` FMAs per chain, the
- * best of `reps` launches after a warm-up. Writes wave64 FMA instructions per second (times 128
- * = FLOP/s). Uses the legacy default stream and allocates for the duration. */
+ * independent non-packed FP32 FMA chains (v_fma_f32 and v_fmac_f32) over 256 wavefronts per CU,
+ * `iters` FMAs per chain, the best of `reps` launches of each form after a warm-up. Writes wave64
+ * FMA instructions per second (times 128 = FLOP/s). Uses the legacy default stream and allocates
+ * for the duration. */
 int asph_probe_valu(int32_t iters, int32_t reps, double* wave_fma_per_s);
 
 /* --- tuning of the neighbour kernels ----------------------------------------------------- */
