@@ -53,8 +53,6 @@ constexpr int BT = 512;            // threads per workgroup (one per brick cell)
 constexpr int BW = BT / 64;        // wavefronts per workgroup
 constexpr int QK = 12;             // queued in-support pairs per thread, momentum
 constexpr int MU = 3;              // queue entries per drain iteration, momentum
-constexpr int DG = 1;              // candidates per step of the density walk (its pair terms are
-                                   // evaluated for every candidate, past a span's end too)
 static_assert(QK % MU == 0 && QK % 2 == 0, "a drain iteration never reads past the queue");
 template <bool MOM> constexpr int queue_len() { return MOM ? QK : 0; }   // density queues nothing
 constexpr int NSLAB = 64;          // most slabs of one halo (beyond: the global-memory loop)
@@ -718,8 +716,13 @@ __device__ __forceinline__ int walk_rows(const DevParams& P, const BrickGeo& g, 
                 e = min(max(e, 0), cnt);
             }
             ncand += e - a;
-            // G candidates per step, fewer at the end of the span: cand(p, n), n = min(G, e - p)
-            for (int p = a; p < e; p += G) cand(p, e - p);
+            // G candidates per step, fewer at the end of the span: cand(p, n), n = min(G, e - p);
+            // G = 1: cand(q) with q the staged candidate's slot, one address stepping the span
+            if constexpr (G == 1) {
+                for (const float4 *q = sm.p + a, *qe = sm.p + e; q < qe; ++q) cand(q);
+            } else {
+                for (int p = a; p < e; p += G) cand(p, e - p);
+            }
         }
     }
     return ncand;
@@ -766,26 +769,22 @@ __device__ __forceinline__ void dens_brick(const DevParams& P, const DensArgs& A
         for (int s = 0; s < ns; ++s) {
             if (ns > 1) stage_slab<S, false>(P, k0, g, s, A.pos, A.cstart, sm, rec);
             if (active) {
-                // one candidate per step (DG). The kernel value is evaluated for every candidate
-                // (its spline is a few operations, cheaper than queueing the in-support ones) and
-                // added where the reference's test r2 < 4h^2 holds, in the reference order; with
-                // more candidates per step, the evaluations past the end of odd spans cost more
-                // than the overlap gains.
-                ncand += walk_rows<S, ROWS, DG>(P, g, sm.delta, sm, s, cc, [&](int p, int n) {
-                    float4 pj[DG];
-#pragma unroll
-                    for (int u = 0; u < DG; ++u) pj[u] = sm.p[u < n ? p + u : p];
-#pragma unroll
-                    for (int u = 0; u < DG; ++u) {
-                        float dxm, dym, dzm;
-                        diff(pj[u], dxm, dym, dzm);
-                        const float r2 = r2of(dxm, dym, dzm);
-                        const bool a = (u < n) & (r2 < P.lim);
-                        const float W = fker_sel(sqrt_rn(r2) * P.inv_h);
-                        S1 = a ? S1 + W : S1;
-                        S2 = a ? S2 + W * pj[u].w : S2;
-                        npair += a ? 1 : 0;
-                    }
+                // one candidate per step, in the reference order. The kernel value is
+                // computed under the reference's test r2 < 4h^2 and added there, with no queue:
+                // the square root and the spline are a branch that a wavefront skips when none
+                // of its lanes has its candidate in support, as at the stencil's outer rows,
+                // which all lanes walk together.
+                ncand += walk_rows<S, ROWS, 1>(P, g, sm.delta, sm, s, cc, [&](const float4* q) {
+                    const float4 pj = *q;
+                    float dxm, dym, dzm;
+                    diff(pj, dxm, dym, dzm);
+                    const float r2 = r2of(dxm, dym, dzm);
+                    const bool a = r2 < P.lim;
+                    // outside the support the sums take +0, which leaves them unchanged
+                    const float W = a ? fker_sel(sqrt_rn(r2) * P.inv_h) : 0.f;   // a branch
+                    S1 += W;
+                    S2 += W * pj.w;
+                    npair += a ? 1 : 0;
                 });
             }
         }
