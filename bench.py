@@ -49,7 +49,7 @@ PROFILED = {
     "window": "steps 6-25 of the 256^3 run",
     "valu_source": "profiles/step_256cubed_valu_counters.md",
     "traffic_source": "profiles/step_256cubed_hbm_traffic.md",
-    "valu_instr": dict(momentum_kdk=3.5939e9, density_eos=2.5804e9),
+    "valu_instr": dict(momentum_kdk=3.5939e9, density_eos=2.0385e9),
     "traffic_bytes": dict(momentum_kdk=2.675e9, density_eos=2.145e9),
 }
 
